@@ -1,0 +1,130 @@
+/*
+ * cm_oracle.h -- CPU oracle for the celestial-mechanics Barnes-Hut hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code.  It is used by tests/, by __graft_entry__.smoke() as the
+ * checker, and by bench.py's cpu_baseline leg (timed as the CPU restatement of
+ * the reference's FP64 path).
+ *
+ * Every function restates either
+ *   (a) the reference's C# FP64 physics path (file:line cited per function), or
+ *   (b) a definition this build adds on top of it (30-bit/63-bit Morton keys,
+ *       stable key sort, the level-order bucket octree, the group-MAC
+ *       traversal, Kahan reductions, the correctly-rounded cube root),
+ * in plain C with IEEE double arithmetic and no FMA contraction
+ * (-ffp-contract=off), so that results are bit-reproducible.
+ */
+#ifndef CM_ORACLE_H
+#define CM_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- gravity ----------------------------------------------------------- */
+double cmo_gravity_coeff(double dist_sq, double src_mass, double src_radius,
+                         double eps2, int shell);
+void cmo_direct_n3(int n, const double* px, const double* py, const double* pz,
+                   const double* m, const uint8_t* act, double eps,
+                   double* ax, double* ay, double* az);
+void cmo_direct_gk(int n, const double* px, const double* py, const double* pz,
+                   const double* m, const double* rad, const uint8_t* act,
+                   double eps, int shell, double* ax, double* ay, double* az);
+void cmo_direct_targets(int n, const double* px, const double* py,
+                        const double* pz, const double* m, const uint8_t* act,
+                        double eps, int nt, const int32_t* targets,
+                        double* ax, double* ay, double* az, int nthreads);
+
+/* ---- reference Barnes-Hut (C# insertion octree) ------------------------ */
+typedef struct {
+    int64_t node_count;
+    int64_t interactions;   /* body-node + body-body terms evaluated */
+    double build_ms;
+    double traverse_ms;
+} cmo_bh_stats;
+int cmo_bh_csharp(int n, const double* px, const double* py, const double* pz,
+                  const double* m, const double* rad, const uint8_t* act,
+                  double eps, double theta, int shell, double min_node_size,
+                  int nthreads, double* ax, double* ay, double* az,
+                  cmo_bh_stats* stats);
+
+/* ---- Verlet ------------------------------------------------------------ */
+void cmo_verlet_kick_drift(int n, const uint8_t* act, double dt,
+                           double* px, double* py, double* pz,
+                           double* vx, double* vy, double* vz,
+                           const double* oax, const double* oay, const double* oaz);
+void cmo_verlet_kick_rotate(int n, const uint8_t* act, double dt,
+                            double* vx, double* vy, double* vz,
+                            const double* ax, const double* ay, const double* az,
+                            double* oax, double* oay, double* oaz);
+
+/* ---- build-defined octree ---------------------------------------------- */
+void cmo_root_box(int n, const double* px, const double* py, const double* pz,
+                  const uint8_t* act, double* box4 /* cx,cy,cz,half */);
+uint64_t cmo_morton_key(double x, double y, double z, const double* box4,
+                        int levels);
+void cmo_morton_keys(int n, const double* px, const double* py,
+                     const double* pz, const uint8_t* act, const double* box4,
+                     int levels, uint64_t* keys);
+void cmo_stable_sort_keys(int n, const uint64_t* keys, uint64_t* keys_sorted,
+                          int32_t* order);
+int cmo_tree_build(int n, const double* px, const double* py, const double* pz,
+                   const double* m, const uint8_t* act, int levels, int leaf_cap,
+                   int node_cap, double* box4, int32_t* n_active,
+                   uint64_t* keys_sorted, int32_t* order,
+                   int32_t* nd_level, int32_t* nd_start, int32_t* nd_count,
+                   int32_t* nd_first, int32_t* nd_nchild, double* nd_mass,
+                   double* nd_com /* 3 per node */, int32_t* level_start);
+int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
+                const double* m, const double* rad, const uint8_t* act,
+                double eps, double theta, int shell, int levels, int leaf_cap,
+                int group_size, int split_depth, int lifo /* stack capacity; 0 = FIFO */, int nthreads,
+                double* ax, double* ay, double* az,
+                int32_t* n_cells, int32_t* n_parts, int32_t* max_queue);
+
+/* ---- diagnostics -------------------------------------------------------- */
+typedef struct {
+    double ke, pe, px, py, pz, lx, ly, lz, l_mag;
+} cmo_energy_t;
+void cmo_energy_naive(int n, const double* px, const double* py,
+                      const double* pz, const double* vx, const double* vy,
+                      const double* vz, const double* m, const uint8_t* act,
+                      double eps, int with_pe, cmo_energy_t* out);
+void cmo_energy_exact(int n, const double* px, const double* py,
+                      const double* pz, const double* vx, const double* vy,
+                      const double* vz, const double* m, const uint8_t* act,
+                      double eps, int with_pe, int nthreads, cmo_energy_t* out);
+typedef struct {
+    double mean, max, rms, median;
+    int32_t count;
+} cmo_ferr_t;
+void cmo_force_error(int n, const uint8_t* act, const double* ex,
+                     const double* ey, const double* ez, const double* tx,
+                     const double* ty, const double* tz, cmo_ferr_t* out);
+
+/* ---- collisions -------------------------------------------------------- */
+typedef struct {
+    int32_t a, b;
+    double overlap, dist, nx, ny, nz, rel_normal_speed;
+} cmo_event_t;
+int cmo_detect(int n, const double* px, const double* py, const double* pz,
+               const double* vx, const double* vy, const double* vz,
+               const double* m, const double* rad, const uint8_t* act,
+               const uint8_t* col, cmo_event_t* out, int cap);
+double cmo_cbrt(double x);
+double cmo_body_radius(double mass, double density, int32_t type);
+int cmo_resolve_merge(int ne, const cmo_event_t* ev, int n, double* px,
+                      double* py, double* pz, double* vx, double* vy,
+                      double* vz, double* ax, double* ay, double* az,
+                      double* oax, double* oay, double* oaz, double* m,
+                      double* rad, double* dens, uint8_t* act, uint8_t* col,
+                      int32_t* type);
+int cmo_compact(int n, double* const* f64_arrays, int n_f64,
+                uint8_t* const* u8_arrays, int n_u8, int32_t* const* i32_arrays,
+                int n_i32, const uint8_t* act, int32_t* old_index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

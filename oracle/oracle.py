@@ -1,0 +1,333 @@
+"""ctypes front end of the CPU oracle (liboracle_cm.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
+checker and by bench.py's cpu_baseline leg.  The product (libcmhip.so and the
+cmhip package) never imports it.
+
+Each wrapper returns numpy arrays; see cm_oracle.c for the reference citations
+of every restated function.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_cm.so")
+_lib = None
+
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+class BHStats(C.Structure):
+    _fields_ = [("node_count", C.c_int64), ("interactions", C.c_int64),
+                ("build_ms", C.c_double), ("traverse_ms", C.c_double)]
+
+
+class Energy(C.Structure):
+    _fields_ = [(k, C.c_double) for k in
+                ("ke", "pe", "px", "py", "pz", "lx", "ly", "lz", "l_mag")]
+
+
+class ForceErr(C.Structure):
+    _fields_ = [("mean", C.c_double), ("max", C.c_double), ("rms", C.c_double),
+                ("median", C.c_double), ("count", C.c_int32)]
+
+
+class Event(C.Structure):
+    _fields_ = [("a", C.c_int32), ("b", C.c_int32), ("overlap", C.c_double),
+                ("dist", C.c_double), ("nx", C.c_double), ("ny", C.c_double),
+                ("nz", C.c_double), ("rel_normal_speed", C.c_double)]
+
+
+EVENT_DTYPE = np.dtype([("a", "<i4"), ("b", "<i4"), ("overlap", "<f8"),
+                        ("dist", "<f8"), ("nx", "<f8"), ("ny", "<f8"),
+                        ("nz", "<f8"), ("rel_normal_speed", "<f8")])
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", _HERE], check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    I, D, V = C.c_int, C.c_double, None
+    L.cmo_gravity_coeff.argtypes = [D, D, D, D, I]
+    L.cmo_gravity_coeff.restype = D
+    L.cmo_direct_n3.argtypes = [I, f64p, f64p, f64p, f64p, u8p, D, f64p, f64p, f64p]
+    L.cmo_direct_n3.restype = V
+    L.cmo_direct_gk.argtypes = [I, f64p, f64p, f64p, f64p, f64p, u8p, D, I,
+                                f64p, f64p, f64p]
+    L.cmo_direct_gk.restype = V
+    L.cmo_direct_targets.argtypes = [I, f64p, f64p, f64p, f64p, u8p, D, I, i32p,
+                                     f64p, f64p, f64p, I]
+    L.cmo_direct_targets.restype = V
+    L.cmo_bh_csharp.argtypes = [I, f64p, f64p, f64p, f64p, f64p, u8p, D, D, I, D,
+                                I, f64p, f64p, f64p, C.POINTER(BHStats)]
+    L.cmo_bh_csharp.restype = I
+    L.cmo_verlet_kick_drift.argtypes = [I, u8p, D, f64p, f64p, f64p, f64p, f64p,
+                                        f64p, f64p, f64p, f64p]
+    L.cmo_verlet_kick_drift.restype = V
+    L.cmo_verlet_kick_rotate.argtypes = [I, u8p, D, f64p, f64p, f64p, f64p, f64p,
+                                         f64p, f64p, f64p, f64p]
+    L.cmo_verlet_kick_rotate.restype = V
+    L.cmo_root_box.argtypes = [I, f64p, f64p, f64p, u8p, f64p]
+    L.cmo_root_box.restype = V
+    L.cmo_morton_keys.argtypes = [I, f64p, f64p, f64p, u8p, f64p, I, u64p]
+    L.cmo_morton_keys.restype = V
+    L.cmo_stable_sort_keys.argtypes = [I, u64p, u64p, i32p]
+    L.cmo_stable_sort_keys.restype = V
+    L.cmo_tree_build.argtypes = [I, f64p, f64p, f64p, f64p, u8p, I, I, I, f64p,
+                                 C.POINTER(C.c_int32), u64p, i32p, i32p, i32p,
+                                 i32p, i32p, i32p, f64p, f64p, i32p]
+    L.cmo_tree_build.restype = I
+    L.cmo_bh_tree.argtypes = [I, f64p, f64p, f64p, f64p, f64p, u8p, D, D, I, I,
+                              I, I, I, I, I, f64p, f64p, f64p, i32p, i32p,
+                              C.POINTER(C.c_int32)]
+    L.cmo_bh_tree.restype = I
+    L.cmo_energy_naive.argtypes = [I, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                                   u8p, D, I, C.POINTER(Energy)]
+    L.cmo_energy_naive.restype = V
+    L.cmo_energy_exact.argtypes = [I, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                                   u8p, D, I, I, C.POINTER(Energy)]
+    L.cmo_energy_exact.restype = V
+    L.cmo_force_error.argtypes = [I, C.c_void_p, f64p, f64p, f64p, f64p, f64p,
+                                  f64p, C.POINTER(ForceErr)]
+    L.cmo_force_error.restype = V
+    L.cmo_detect.argtypes = [I, f64p, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
+                             u8p, u8p, C.c_void_p, I]
+    L.cmo_detect.restype = I
+    L.cmo_cbrt.argtypes = [D]
+    L.cmo_cbrt.restype = D
+    L.cmo_body_radius.argtypes = [D, D, C.c_int32]
+    L.cmo_body_radius.restype = D
+    L.cmo_resolve_merge.argtypes = [I, C.c_void_p, I] + [f64p] * 15 + [u8p, u8p, i32p]
+    L.cmo_resolve_merge.restype = I
+    _lib = L
+    return L
+
+
+# ---------------------------------------------------------------------------
+# state helpers: a "state" is a dict of numpy arrays keyed like BodySoA fields
+# ---------------------------------------------------------------------------
+F64_FIELDS = ("px", "py", "pz", "vx", "vy", "vz", "ax", "ay", "az",
+              "oax", "oay", "oaz", "mass", "radius", "density")
+U8_FIELDS = ("active", "collidable")
+I32_FIELDS = ("body_type",)
+
+
+def copy_state(s):
+    return {k: np.array(v, copy=True) for k, v in s.items()}
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def gravity_coeff(dist_sq, m, r, eps2, shell):
+    return lib().cmo_gravity_coeff(dist_sq, m, r, eps2, int(shell))
+
+
+def direct_n3(s, eps):
+    n = len(s["px"])
+    ax, ay, az = (np.zeros(n) for _ in range(3))
+    lib().cmo_direct_n3(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                        _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                        _c(s["active"], np.uint8), eps, ax, ay, az)
+    return ax, ay, az
+
+
+def direct_gk(s, eps, shell=False):
+    n = len(s["px"])
+    ax, ay, az = (np.zeros(n) for _ in range(3))
+    lib().cmo_direct_gk(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                        _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                        _c(s["radius"], np.float64), _c(s["active"], np.uint8),
+                        eps, int(shell), ax, ay, az)
+    return ax, ay, az
+
+
+def direct_targets(s, eps, targets, nthreads=0):
+    t = _c(targets, np.int32)
+    ax, ay, az = (np.zeros(len(t)) for _ in range(3))
+    n = len(s["px"])
+    lib().cmo_direct_targets(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                             _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                             _c(s["active"], np.uint8), eps, len(t), t,
+                             ax, ay, az, nthreads)
+    return ax, ay, az
+
+
+def bh_csharp(s, eps, theta, shell=False, min_node_size=1e-12, nthreads=0):
+    n = len(s["px"])
+    ax, ay, az = (np.zeros(n) for _ in range(3))
+    st = BHStats()
+    rc = lib().cmo_bh_csharp(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                             _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                             _c(s["radius"], np.float64), _c(s["active"], np.uint8),
+                             eps, theta, int(shell), min_node_size, nthreads,
+                             ax, ay, az, C.byref(st))
+    if rc != 0:
+        raise MemoryError("oracle BH pool allocation failed")
+    return (ax, ay, az), st
+
+
+def verlet_step(s, dt, force_fn):
+    """SoAVerletIntegrator.Step (Integrators/SoAVerletIntegrator.cs:48-115) on a
+    state dict in place; force_fn(state) -> (ax, ay, az)."""
+    L = lib()
+    n = len(s["px"])
+    L.cmo_verlet_kick_drift(n, s["active"], dt, s["px"], s["py"], s["pz"],
+                            s["vx"], s["vy"], s["vz"], s["oax"], s["oay"], s["oaz"])
+    ax, ay, az = force_fn(s)
+    s["ax"][:] = ax
+    s["ay"][:] = ay
+    s["az"][:] = az
+    L.cmo_verlet_kick_rotate(n, s["active"], dt, s["vx"], s["vy"], s["vz"],
+                             s["ax"], s["ay"], s["az"], s["oax"], s["oay"], s["oaz"])
+
+
+def root_box(s):
+    n = len(s["px"])
+    box = np.zeros(4)
+    lib().cmo_root_box(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                       _c(s["pz"], np.float64), _c(s["active"], np.uint8), box)
+    return box
+
+
+def morton_keys(s, box, levels):
+    n = len(s["px"])
+    keys = np.zeros(n, dtype=np.uint64)
+    lib().cmo_morton_keys(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                          _c(s["pz"], np.float64), _c(s["active"], np.uint8),
+                          _c(box, np.float64), levels, keys)
+    return keys
+
+
+def stable_sort_keys(keys):
+    n = len(keys)
+    ks = np.zeros(n, dtype=np.uint64)
+    order = np.zeros(n, dtype=np.int32)
+    lib().cmo_stable_sort_keys(n, _c(keys, np.uint64), ks, order)
+    return ks, order
+
+
+def tree_build(s, levels, leaf_cap):
+    n = len(s["px"])
+    cap = 2 * n + 16
+    box = np.zeros(4)
+    na = C.c_int32(0)
+    ks = np.zeros(n, dtype=np.uint64)
+    order = np.zeros(n, dtype=np.int32)
+    arrs = [np.zeros(cap, dtype=np.int32) for _ in range(5)]
+    mass = np.zeros(cap)
+    com = np.zeros(3 * cap)
+    level_start = np.zeros(levels + 3, dtype=np.int32)
+    nn = lib().cmo_tree_build(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                              _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                              _c(s["active"], np.uint8), levels, leaf_cap, cap, box,
+                              C.byref(na), ks, order, *arrs, mass, com, level_start)
+    if nn < 0:
+        raise RuntimeError("node capacity exceeded")
+    return {
+        "box": box, "n_active": na.value, "keys": ks, "order": order,
+        "level": arrs[0][:nn], "start": arrs[1][:nn], "count": arrs[2][:nn],
+        "first": arrs[3][:nn], "nchild": arrs[4][:nn], "mass": mass[:nn],
+        "com": com[:3 * nn].reshape(nn, 3), "level_start": level_start,
+        "n_nodes": nn,
+    }
+
+
+def bh_tree(s, eps, theta, levels=10, leaf_cap=16, group_size=64, shell=False,
+            split_depth=0, lifo=False, nthreads=0):
+    n = len(s["px"])
+    ax, ay, az = (np.zeros(n) for _ in range(3))
+    nc = np.zeros(n, dtype=np.int32)
+    npart = np.zeros(n, dtype=np.int32)
+    mq = C.c_int32(0)
+    lib().cmo_bh_tree(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                      _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                      _c(s["radius"], np.float64), _c(s["active"], np.uint8),
+                      eps, theta, int(shell), levels, leaf_cap, group_size,
+                      split_depth, int(lifo), nthreads, ax, ay, az, nc, npart,
+                      C.byref(mq))
+    return (ax, ay, az), {"n_cells": nc, "n_parts": npart, "max_queue": mq.value}
+
+
+def energy_naive(s, eps, with_pe=True):
+    e = Energy()
+    n = len(s["px"])
+    lib().cmo_energy_naive(n, *(_c(s[k], np.float64) for k in
+                                ("px", "py", "pz", "vx", "vy", "vz", "mass")),
+                           _c(s["active"], np.uint8), eps, int(with_pe), C.byref(e))
+    return {k: getattr(e, k) for k, _ in Energy._fields_}
+
+
+def energy_exact(s, eps, with_pe=True, nthreads=0):
+    e = Energy()
+    n = len(s["px"])
+    lib().cmo_energy_exact(n, *(_c(s[k], np.float64) for k in
+                                ("px", "py", "pz", "vx", "vy", "vz", "mass")),
+                           _c(s["active"], np.uint8), eps, int(with_pe), nthreads,
+                           C.byref(e))
+    return {k: getattr(e, k) for k, _ in Energy._fields_}
+
+
+def force_error(exact, test, active=None):
+    r = ForceErr()
+    n = len(exact[0])
+    act = None if active is None else _c(active, np.uint8)
+    lib().cmo_force_error(n, None if act is None else act.ctypes.data,
+                          *(_c(a, np.float64) for a in exact),
+                          *(_c(a, np.float64) for a in test), C.byref(r))
+    return {"mean": r.mean, "max": r.max, "rms": r.rms, "median": r.median,
+            "count": r.count}
+
+
+def detect(s, cap=1 << 20):
+    n = len(s["px"])
+    out = np.zeros(cap, dtype=EVENT_DTYPE)
+    ne = lib().cmo_detect(n, *(_c(s[k], np.float64) for k in
+                               ("px", "py", "pz", "vx", "vy", "vz", "mass", "radius")),
+                          _c(s["active"], np.uint8), _c(s["collidable"], np.uint8),
+                          out.ctypes.data, cap)
+    if ne > cap:
+        raise RuntimeError("event capacity exceeded")
+    return out[:ne].copy()
+
+
+def cbrt(x):
+    return lib().cmo_cbrt(float(x))
+
+
+def body_radius(mass, density, body_type):
+    return lib().cmo_body_radius(float(mass), float(density), int(body_type))
+
+
+def resolve_merge(events, s):
+    """In-place MergeOnly resolution; returns merge count."""
+    ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+    n = len(s["px"])
+    return lib().cmo_resolve_merge(len(ev), ev.ctypes.data, n,
+                                   *(s[k] for k in F64_FIELDS),
+                                   s["active"], s["collidable"], s["body_type"])
+
+
+def compact(s):
+    """Order-preserving compaction of a state dict; returns (state, old_index)."""
+    act = s["active"].astype(bool)
+    idx = np.nonzero(act)[0].astype(np.int32)
+    out = {k: np.ascontiguousarray(v[idx]) for k, v in s.items()}
+    return out, idx
