@@ -1,0 +1,367 @@
+"""Python host side of libcmhip.so, mirroring the reference's C# physics surface.
+
+Class and method names follow src/CelestialMechanics.Physics of
+mfscpayload-690/celestial-mechanics-desktop so that tests read like the
+reference's own:
+
+  BodySoA                 SoA/BodySoA.cs:39-175
+  HipPhysicsBackend       IPhysicsComputeBackend (Solvers/IPhysicsComputeBackend.cs:21-33)
+                          + IGravityModelAwareBackend (Solvers/IGravityModelAwareBackend.cs:6-9),
+                          in the slot of CudaPhysicsBackend (Solvers/CudaPhysicsBackend.cs:35-48)
+  SoAVerletIntegrator     Integrators/SoAVerletIntegrator.cs:48-115
+  EnergyCalculator        Validation/EnergyCalculator.cs:16-81
+  CollisionDetector       Collisions/CollisionDetector.cs:35-135
+  CollisionResolver       Collisions/CollisionResolver.cs:58-345 (MergeOnly)
+
+Every computation runs in libcmhip.so on the GPU.  There is no CPU fallback:
+if the library is missing or no GPU is present the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libcmhip.so")
+
+# ---- constants of include/cmhip.h ----------------------------------------
+CM_OK = 0
+ERRORS = {-1: "HipError", -2: "OutOfMemory", -3: "InvalidArgument",
+          -4: "PoolExhausted", -5: "DeviceNotFound", -6: "NotInitialized"}
+BACKEND_DIRECT_PAIRWISE = 0
+BACKEND_DIRECT = 1
+BACKEND_BARNES_HUT = 2
+PREC_FP64 = 0
+PREC_FP64_FAST = 1
+PREC_FP32_SPLIT = 2
+
+F64_FIELDS = ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z",
+              "acc_x", "acc_y", "acc_z", "old_acc_x", "old_acc_y", "old_acc_z",
+              "mass", "radius", "density")
+U8_FIELDS = ("is_active", "is_collidable")
+I32_FIELDS = ("body_type",)
+ALL_FIELDS = F64_FIELDS + U8_FIELDS + I32_FIELDS
+
+# Names exported by libcmhip.so, as declared in include/cmhip.h.
+EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
+           "cm_default_config", "cm_upload_soa", "cm_download_soa",
+           "cm_get_count", "cm_compute_forces", "cm_verlet_step", "cm_energy",
+           "cm_direct_sample", "cm_get_force_stats", "cm_detect_collisions",
+           "cm_resolve_merges", "cm_compact", "cm_generate_plummer",
+           "cm_reset_force_stats")
+
+
+class CmError(RuntimeError):
+    def __init__(self, code, message=""):
+        self.code = code
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+
+
+class BackendConfig(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("precision", C.c_int32),
+                ("shell_theorem", C.c_int32), ("morton_levels", C.c_int32),
+                ("theta", C.c_double), ("leaf_capacity", C.c_int32),
+                ("group_split_depth", C.c_int32)]
+
+
+class SoAPtrs(C.Structure):
+    _fields_ = ([(f, C.POINTER(C.c_double)) for f in F64_FIELDS]
+                + [(f, C.POINTER(C.c_uint8)) for f in U8_FIELDS]
+                + [("body_type", C.POINTER(C.c_int32))])
+
+
+class ForceStats(C.Structure):
+    _fields_ = [("cell_interactions", C.c_int64), ("body_interactions", C.c_int64),
+                ("n_active", C.c_int32), ("n_nodes", C.c_int32),
+                ("n_groups", C.c_int32), ("flags", C.c_int32),
+                ("ms_tree", C.c_float), ("ms_sort", C.c_float),
+                ("ms_force", C.c_float), ("ms_total", C.c_float),
+                ("acc_interactions", C.c_int64), ("acc_ms_force", C.c_double),
+                ("acc_ms_total", C.c_double), ("acc_evaluations", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class EnergyReport(C.Structure):
+    _fields_ = [("kinetic", C.c_double), ("potential", C.c_double),
+                ("momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
+                ("angular_momentum_mag", C.c_double), ("total_mass", C.c_double),
+                ("active_count", C.c_int32)]
+
+
+EVENT_DTYPE = np.dtype([("body_a", "<i4"), ("body_b", "<i4"),
+                        ("overlap_depth", "<f8"), ("distance", "<f8"),
+                        ("normal_x", "<f8"), ("normal_y", "<f8"),
+                        ("normal_z", "<f8"), ("relative_normal_speed", "<f8")])
+
+_lib = None
+
+
+def load_library():
+    """Load libcmhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CmError(-6, f"{LIB_PATH} not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, f64 = C.c_void_p, C.c_int32, C.c_double
+    sig = {
+        "cm_create": ([i32, i32, C.POINTER(vp)], i32),
+        "cm_destroy": ([vp], i32),
+        "cm_last_error": ([vp], C.c_char_p),
+        "cm_configure": ([vp, C.POINTER(BackendConfig)], i32),
+        "cm_default_config": ([C.POINTER(BackendConfig)], i32),
+        "cm_upload_soa": ([vp, C.POINTER(SoAPtrs), i32], i32),
+        "cm_download_soa": ([vp, C.POINTER(SoAPtrs), i32], i32),
+        "cm_get_count": ([vp, C.POINTER(i32)], i32),
+        "cm_compute_forces": ([vp, f64], i32),
+        "cm_verlet_step": ([vp, f64, f64, i32], i32),
+        "cm_energy": ([vp, f64, i32, C.POINTER(EnergyReport)], i32),
+        "cm_direct_sample": ([vp, f64, vp, i32, vp, vp, vp], i32),
+        "cm_get_force_stats": ([vp, C.POINTER(ForceStats)], i32),
+        "cm_reset_force_stats": ([vp], i32),
+        "cm_detect_collisions": ([vp, vp, i32, C.POINTER(i32)], i32),
+        "cm_resolve_merges": ([vp, C.POINTER(i32)], i32),
+        "cm_compact": ([vp, C.POINTER(i32), vp], i32),
+        "cm_generate_plummer": ([i32, f64, f64, C.c_uint64, f64, i32,
+                                 C.POINTER(SoAPtrs)], i32),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+class BodySoA:
+    """Structure-of-arrays body store (SoA/BodySoA.cs:39-175): 15 float64
+    arrays, IsActive / IsCollidable as uint8 and BodyTypeIndex as int32."""
+
+    def __init__(self, capacity):
+        if capacity <= 0:
+            raise ValueError("capacity")   # BodySoA.cs:83
+        self.capacity = int(capacity)
+        self.count = 0
+        for f in F64_FIELDS:
+            setattr(self, f, np.zeros(capacity, dtype=np.float64))
+        for f in U8_FIELDS:
+            setattr(self, f, np.zeros(capacity, dtype=np.uint8))
+        self.body_type = np.zeros(capacity, dtype=np.int32)
+
+    @classmethod
+    def from_arrays(cls, count=None, capacity=None, **arrays):
+        n = count if count is not None else len(arrays["pos_x"])
+        soa = cls(capacity or max(n, 1))
+        soa.count = n
+        for k, v in arrays.items():
+            getattr(soa, k)[:n] = v
+        if "old_acc_x" not in arrays:   # CopyFrom seeds OldAcc = Acc (BodySoA.cs:141-144)
+            soa.old_acc_x[:n] = soa.acc_x[:n]
+            soa.old_acc_y[:n] = soa.acc_y[:n]
+            soa.old_acc_z[:n] = soa.acc_z[:n]
+        return soa
+
+    def copy(self):
+        out = BodySoA(self.capacity)
+        out.count = self.count
+        for f in ALL_FIELDS:
+            getattr(out, f)[:] = getattr(self, f)
+        return out
+
+    def ptrs(self):
+        p = SoAPtrs()
+        for f in F64_FIELDS:
+            setattr(p, f, _ptr(getattr(self, f), C.c_double))
+        for f in U8_FIELDS:
+            setattr(p, f, _ptr(getattr(self, f), C.c_uint8))
+        p.body_type = _ptr(self.body_type, C.c_int32)
+        return p
+
+
+def generate_plummer(count, total_mass=1.0, scale_length=1.0, seed=42,
+                     mass_fraction_cut=1.0, com_frame=True, capacity=None):
+    """PlummerSphere.Generate (ScenarioGenerators/PlummerSphere.cs:41-116) with a
+    SplitMix64 stream; optional truncation at a mass fraction; all bodies
+    active and non-collidable, radius 0.02."""
+    soa = BodySoA(capacity or max(count, 1))
+    rc = load_library().cm_generate_plummer(count, total_mass, scale_length,
+                                            C.c_uint64(seed), mass_fraction_cut,
+                                            int(com_frame), C.byref(soa.ptrs()))
+    if rc != CM_OK:
+        raise CmError(rc, "cm_generate_plummer")
+    soa.count = count
+    soa.is_active[:count] = 1
+    return soa
+
+
+class HipPhysicsBackend:
+    """IPhysicsComputeBackend + IGravityModelAwareBackend on the GPU.
+
+    compute_forces(bodies, softening) follows the C# contract exactly: it
+    reads positions / masses / radii / active flags of `bodies`, and
+    overwrites AccX/Y/Z for every index < Count (inactive bodies get 0).
+    For device-resident stepping use upload() / step() / download()."""
+
+    def __init__(self, capacity, kind=BACKEND_BARNES_HUT, precision=PREC_FP64,
+                 theta=0.5, morton_levels=10, leaf_capacity=16,
+                 group_split_depth=3, enable_shell_theorem=False, device=0):
+        L = load_library()
+        h = C.c_void_p()
+        rc = L.cm_create(int(capacity), int(device), C.byref(h))
+        if rc != CM_OK:
+            raise CmError(rc, "cm_create")
+        self._h = h
+        self._L = L
+        self.capacity = int(capacity)
+        self.config = BackendConfig()
+        L.cm_default_config(C.byref(self.config))
+        self.config.kind = kind
+        self.config.precision = precision
+        self.config.theta = theta
+        self.config.morton_levels = morton_levels
+        self.config.leaf_capacity = leaf_capacity
+        self.config.group_split_depth = group_split_depth
+        self.config.shell_theorem = int(enable_shell_theorem)
+        self._apply()
+
+    # -- configuration ----------------------------------------------------
+    def _check(self, rc, what):
+        if rc != CM_OK:
+            msg = self._L.cm_last_error(self._h)
+            raise CmError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def _apply(self):
+        self._check(self._L.cm_configure(self._h, C.byref(self.config)), "cm_configure")
+
+    def configure(self, **kw):
+        for k, v in kw.items():
+            if k == "enable_shell_theorem":
+                k, v = "shell_theorem", int(v)
+            setattr(self.config, k, v)
+        self._apply()
+
+    @property
+    def enable_shell_theorem(self):
+        return bool(self.config.shell_theorem)
+
+    @enable_shell_theorem.setter
+    def enable_shell_theorem(self, v):
+        self.configure(shell_theorem=int(bool(v)))
+
+    @property
+    def theta(self):
+        return self.config.theta
+
+    @theta.setter
+    def theta(self, v):
+        self.configure(theta=float(v))
+
+    # -- IPhysicsComputeBackend ---------------------------------------------
+    def compute_forces(self, bodies, softening):
+        self.upload(bodies)
+        self._check(self._L.cm_compute_forces(self._h, float(softening)), "cm_compute_forces")
+        self.download(bodies, fields=("acc_x", "acc_y", "acc_z"))
+
+    # -- device-resident interface ----------------------------------------
+    def upload(self, bodies):
+        self._check(self._L.cm_upload_soa(self._h, C.byref(bodies.ptrs()), bodies.count),
+                    "cm_upload_soa")
+
+    def download(self, bodies, fields=ALL_FIELDS):
+        p = SoAPtrs()
+        for f in fields:
+            a = getattr(bodies, f)
+            ct = C.c_double if f in F64_FIELDS else (C.c_uint8 if f in U8_FIELDS else C.c_int32)
+            setattr(p, f, _ptr(a, ct))
+        n = self.count
+        bodies.count = n
+        self._check(self._L.cm_download_soa(self._h, C.byref(p), n), "cm_download_soa")
+
+    @property
+    def count(self):
+        c = C.c_int32()
+        self._check(self._L.cm_get_count(self._h, C.byref(c)), "cm_get_count")
+        return c.value
+
+    def compute_forces_resident(self, softening):
+        self._check(self._L.cm_compute_forces(self._h, float(softening)), "cm_compute_forces")
+
+    def step(self, softening, dt, n_steps=1):
+        self._check(self._L.cm_verlet_step(self._h, float(softening), float(dt), int(n_steps)),
+                    "cm_verlet_step")
+
+    def stats(self):
+        s = ForceStats()
+        self._check(self._L.cm_get_force_stats(self._h, C.byref(s)), "cm_get_force_stats")
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(self._L.cm_reset_force_stats(self._h), "cm_reset_force_stats")
+
+    def energy(self, softening, with_potential=True):
+        r = EnergyReport()
+        self._check(self._L.cm_energy(self._h, float(softening), int(with_potential), C.byref(r)),
+                    "cm_energy")
+        return {"kinetic": r.kinetic, "potential": r.potential,
+                "momentum": tuple(r.momentum), "angular_momentum": tuple(r.angular_momentum),
+                "angular_momentum_mag": r.angular_momentum_mag,
+                "total_mass": r.total_mass, "active_count": r.active_count}
+
+    def direct_sample(self, softening, targets):
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        ax, ay, az = (np.zeros(len(t)) for _ in range(3))
+        self._check(self._L.cm_direct_sample(self._h, float(softening), t.ctypes.data, len(t),
+                                             ax.ctypes.data, ay.ctypes.data, az.ctypes.data),
+                    "cm_direct_sample")
+        return ax, ay, az
+
+    def detect_collisions(self, cap=1 << 16):
+        out = np.zeros(cap, dtype=EVENT_DTYPE)
+        ne = C.c_int32()
+        self._check(self._L.cm_detect_collisions(self._h, out.ctypes.data, cap, C.byref(ne)),
+                    "cm_detect_collisions")
+        return out[:min(ne.value, cap)].copy(), ne.value
+
+    def resolve_merges(self):
+        n = C.c_int32()
+        self._check(self._L.cm_resolve_merges(self._h, C.byref(n)), "cm_resolve_merges")
+        return n.value
+
+    def compact(self):
+        n0 = self.count
+        old = np.zeros(max(n0, 1), dtype=np.int32)
+        n = C.c_int32()
+        self._check(self._L.cm_compact(self._h, C.byref(n), old.ctypes.data), "cm_compact")
+        return n.value, old[:n.value].copy()
+
+    def close(self):
+        if self._h:
+            self._L.cm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SoAVerletIntegrator:
+    """ISoAIntegrator.Step (Integrators/SoAVerletIntegrator.cs:48-115) with the
+    HIP backend: upload, one device-resident KDK step, download."""
+
+    name = "Verlet-SoA"
+
+    def step(self, bodies, backend, softening, dt):
+        backend.upload(bodies)
+        backend.step(softening, dt, 1)
+        backend.download(bodies)

@@ -1,0 +1,126 @@
+// cm_internal.h -- device state and kernel entry points of libcmhip.so.
+//
+// Everything here is MI355X/gfx950 code: 64-lane wavefronts (__ballot returns
+// a 64-bit mask), HIP streams of default priority, hipMalloc'd SoA arrays.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/cmhip.h"
+
+namespace cm {
+
+constexpr int kWave = 64;
+constexpr int kMaxLevels = 21;          // 63-bit keys
+constexpr int kStackCap = 2048;         // per-wave LDS traversal stack (ints)
+constexpr int kSortTile = 2048;         // keys per radix-sort tile
+constexpr int kSortThreads = 256;
+constexpr int kBuildTile = 256;         // nodes per tree-build tile
+
+// SoA field order of BodySoA (SoA/BodySoA.cs:42-66).
+enum Field : int {
+    PX, PY, PZ, VX, VY, VZ, AX, AY, AZ, OAX, OAY, OAZ, MASS, RAD, DENS, NF64
+};
+
+// Device-resident per-evaluation header written and read by the kernels so
+// that the host never needs the body count or level sizes of the tree.
+struct TreeHeader {
+    double box[4];             // root cube: centre x, y, z and half size
+    int32_t n_active;
+    int32_t n_nodes;
+    int32_t overflow;
+    int32_t n_levels_used;
+    int32_t level_start[kMaxLevels + 3];
+    uint32_t tile_counter[kMaxLevels + 2];
+    unsigned long long n_cell_int;
+    unsigned long long n_body_int;
+    unsigned long long acc_int;   // cell + body terms since the last stats reset
+};
+
+constexpr int kWindow = 1024;     // force evaluations timed per host sync
+
+struct Engine {
+    int device = 0;
+    int capacity = 0;
+    int count = 0;
+    hipStream_t stream = nullptr;
+    cm_backend_config cfg{};
+    std::string err;
+
+    // BodySoA arrays (device)
+    double* f[NF64] = {};
+    uint8_t* act = nullptr;
+    uint8_t* col = nullptr;
+    int32_t* type = nullptr;
+
+    // Barnes-Hut scratch
+    uint64_t* keys_a = nullptr;
+    uint64_t* keys_b = nullptr;
+    int32_t* vals_a = nullptr;
+    int32_t* vals_b = nullptr;
+    uint32_t* hist = nullptr;          // 256 x sort tiles
+    double* sorted = nullptr;          // 5 x capacity: x, y, z, m, radius
+    double* bbox_part = nullptr;       // per-block min/max partials
+    int32_t* bbox_cnt = nullptr;
+    TreeHeader* hdr = nullptr;
+    TreeHeader* hdr_host = nullptr;   // pinned copy, refreshed after each evaluation
+    int node_cap = 0;
+    int4* nd_info = nullptr;           // first, nchild | level << 8, start, count
+    double4* nd_cm = nullptr;          // com x, y, z, mass
+    double* nd_sum = nullptr;          // 3 x node_cap: sum m*x, m*y, m*z
+    unsigned long long* lookback = nullptr;  // (levels + 1) x build tiles
+
+    // collisions / compaction scratch
+    void* coll = nullptr;
+
+    // diagnostics scratch
+    double* red = nullptr;             // reduction partials
+    double* sample = nullptr;          // direct-sample partials
+
+    // timing
+    bool timing = true;
+    hipEvent_t ev[6] = {};
+    cm_force_stats stats{};
+    // per-evaluation (start, force-kernel start, end) events since the last
+    // host sync, folded into the accumulators by finalize_stats
+    hipEvent_t* wev = nullptr;        // 3 x kWindow
+    int wn = 0;
+    double acc_ms_force = 0.0, acc_ms_total = 0.0;
+    int acc_evals = 0;
+
+    // records the three window events of one force evaluation
+    void mark(int which) {
+        if (wn < kWindow) (void)hipEventRecord(wev[3 * wn + which], stream);
+        if (which == 2 && wn < kWindow) wn++;
+    }
+};
+
+// ---- kernels launchers (one .hip file each) --------------------------------
+int launch_kick_drift(Engine& e, double dt);
+int launch_kick_rotate(Engine& e, double dt);
+int launch_direct(Engine& e, double softening);
+int launch_barnes_hut(Engine& e, double softening);
+int launch_direct_sample(Engine& e, double softening, const int32_t* targets,
+                         int32_t nt, double* ax, double* ay, double* az);
+int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out);
+int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne);
+int collide_resolve(Engine& e, int32_t* merged);
+int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index);
+void collide_free(Engine& e);
+
+int set_error(Engine& e, int code, const std::string& msg);
+int finalize_stats(Engine& e);   // syncs the stream, fills e.stats
+
+}  // namespace cm
+
+#define CM_HIP_TRY(e, call)                                                    \
+    do {                                                                       \
+        hipError_t _st = (call);                                               \
+        if (_st != hipSuccess)                                                 \
+            return ::cm::set_error((e), CM_ERR_HIP,                            \
+                                   std::string(#call) + ": " +                 \
+                                       hipGetErrorString(_st));                \
+    } while (0)

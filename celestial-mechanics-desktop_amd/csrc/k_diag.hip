@@ -1,0 +1,175 @@
+// k_diag.hip -- energy / momentum diagnostics with compensated FP64 sums.
+//
+// Quantities of EnergyCalculator (Validation/EnergyCalculator.cs:16-81):
+// KE = sum 0.5 m |v|^2, P = sum v m, L = sum r x (v m) over active bodies and
+// PE = -sum_{i<j} m_i m_j / sqrt(|r_j - r_i|^2 + eps^2)
+// (Forces/NewtonianGravity.cs:68-77).  Every partial sum is carried as a
+// (sum, compensation) pair and pairs are merged with Neumaier's two-sum rule
+// (the device-side counterpart of the reference's Kahan finalisation,
+// engine/src/cuda/gpu_reduction.cu:21-39), in a fixed launch shape, so the
+// results are deterministic and accurate to a few ulps of the exact sums.
+// PE is formed per body as a full row over j != i (tiled through LDS) and
+// halved at the end.
+#include <hip/hip_runtime.h>
+
+#include "cm_internal.h"
+
+namespace cm {
+
+constexpr int kRedBlocks = 512;
+constexpr int kQ = 10;   // ke, px, py, pz, lx, ly, lz, mass, pe, count
+
+struct Comp {
+    double s, c;
+};
+
+__device__ __forceinline__ void comp_add(Comp& a, double x) {
+    double t = a.s + x;
+    double err = fabs(a.s) >= fabs(x) ? (a.s - t) + x : (x - t) + a.s;
+    a.c += err;
+    a.s = t;
+}
+
+__device__ __forceinline__ Comp comp_merge(Comp a, Comp b) {
+    double t = a.s + b.s;
+    double err = fabs(a.s) >= fabs(b.s) ? (a.s - t) + b.s : (b.s - t) + a.s;
+    Comp r;
+    r.s = t;
+    r.c = a.c + b.c + err;
+    return r;
+}
+
+__device__ void block_reduce_store(Comp (&v)[kQ], Comp* __restrict__ out) {
+    __shared__ double ss[kQ][256];
+    __shared__ double sc[kQ][256];
+    int t = threadIdx.x;
+    for (int q = 0; q < kQ; q++) { ss[q][t] = v[q].s; sc[q][t] = v[q].c; }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) {
+            for (int q = 0; q < kQ; q++) {
+                Comp a{ss[q][t], sc[q][t]}, b{ss[q][t + o], sc[q][t + o]};
+                Comp r = comp_merge(a, b);
+                ss[q][t] = r.s; sc[q][t] = r.c;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < kQ; q++) out[(size_t)q * kRedBlocks + blockIdx.x] = Comp{ss[q][0], sc[q][0]};
+}
+
+__global__ void __launch_bounds__(256)
+k_energy_bodies(int n, const double* __restrict__ px, const double* __restrict__ py,
+                const double* __restrict__ pz, const double* __restrict__ vx,
+                const double* __restrict__ vy, const double* __restrict__ vz,
+                const double* __restrict__ m, const uint8_t* __restrict__ act,
+                const double* __restrict__ pe_rows, Comp* __restrict__ out) {
+    Comp v[kQ];
+    for (int q = 0; q < kQ; q++) v[q] = Comp{0.0, 0.0};
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        if (!act[i]) continue;
+        double mi = m[i];
+        double x = vx[i], y = vy[i], z = vz[i];
+        comp_add(v[0], 0.5 * mi * (x * x + y * y + z * z));
+        double mvx = x * mi, mvy = y * mi, mvz = z * mi;
+        comp_add(v[1], mvx); comp_add(v[2], mvy); comp_add(v[3], mvz);
+        double rx = px[i], ry = py[i], rz = pz[i];
+        comp_add(v[4], ry * mvz - rz * mvy);
+        comp_add(v[5], rz * mvx - rx * mvz);
+        comp_add(v[6], rx * mvy - ry * mvx);
+        comp_add(v[7], mi);
+        if (pe_rows) comp_add(v[8], pe_rows[i]);
+        v[9].s += 1.0;
+    }
+    block_reduce_store(v, out);
+}
+
+__global__ void __launch_bounds__(256)
+k_energy_final(int nblocks, const Comp* __restrict__ part, double* __restrict__ res) {
+    Comp v[kQ];
+    for (int q = 0; q < kQ; q++) {
+        v[q] = Comp{0.0, 0.0};
+        for (int b = threadIdx.x; b < nblocks; b += 256) v[q] = comp_merge(v[q], part[(size_t)q * kRedBlocks + b]);
+    }
+    block_reduce_store(v, reinterpret_cast<Comp*>(res + 2 * kQ));
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int q = 0; q < kQ; q++) {
+            Comp c = reinterpret_cast<Comp*>(res + 2 * kQ)[(size_t)q * kRedBlocks];
+            res[q] = c.s + c.c;
+        }
+}
+
+// Row sums of the pair potential: row_i = sum_{j != i, active} -m_i m_j / d_ij.
+__global__ void __launch_bounds__(256)
+k_pe_rows(int n, double eps2, const double* __restrict__ px,
+          const double* __restrict__ py, const double* __restrict__ pz,
+          const double* __restrict__ m, const uint8_t* __restrict__ act,
+          double* __restrict__ rows) {
+    __shared__ double tx[256], ty[256], tz[256], tm[256];
+    int i = blockIdx.x * 256 + threadIdx.x;
+    bool me = i < n && act[i];
+    double xi = 0, yi = 0, zi = 0, mi = 0;
+    if (i < n) { xi = px[i]; yi = py[i]; zi = pz[i]; mi = m[i]; }
+    Comp r{0.0, 0.0};
+    for (int base = 0; base < n; base += 256) {
+        __syncthreads();
+        int j = base + threadIdx.x;
+        if (j < n) {
+            tx[threadIdx.x] = px[j]; ty[threadIdx.x] = py[j]; tz[threadIdx.x] = pz[j];
+            tm[threadIdx.x] = act[j] ? m[j] : 0.0;
+        } else {
+            tm[threadIdx.x] = 0.0;
+            tx[threadIdx.x] = 0.0; ty[threadIdx.x] = 0.0; tz[threadIdx.x] = 0.0;
+        }
+        __syncthreads();
+        if (!me) continue;
+        int lim = min(256, n - base);
+        double s = 0.0;
+        for (int q = 0; q < lim; q++) {
+            if (base + q == i || tm[q] == 0.0) continue;
+            double dx = tx[q] - xi, dy = ty[q] - yi, dz = tz[q] - zi;
+            double d = sqrt(dx * dx + dy * dy + dz * dz + eps2);
+            s += -1.0 * mi * tm[q] / d;
+        }
+        comp_add(r, s);
+    }
+    if (i < n) rows[i] = me ? 0.5 * (r.s + r.c) : 0.0;
+}
+
+int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out) {
+    const int n = e.count;
+    double* rows = nullptr;
+    hipStream_t st = e.stream;
+    if (with_pe && n > 0) {
+        CM_HIP_TRY(e, hipMalloc(&rows, (size_t)n * 8));
+        hipLaunchKernelGGL(k_pe_rows, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                           softening * softening, e.f[PX], e.f[PY], e.f[PZ],
+                           e.f[MASS], e.act, rows);
+    }
+    Comp* part = reinterpret_cast<Comp*>(e.red);
+    double* res = e.red + 2 * kQ * kRedBlocks;
+    hipLaunchKernelGGL(k_energy_bodies, dim3(kRedBlocks), dim3(256), 0, st, n,
+                       e.f[PX], e.f[PY], e.f[PZ], e.f[VX], e.f[VY], e.f[VZ],
+                       e.f[MASS], e.act, rows, part);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, kRedBlocks, part, res);
+    hipError_t err = hipGetLastError();
+    double h[kQ] = {0};
+    if (err == hipSuccess) err = hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (err == hipSuccess) err = hipStreamSynchronize(st);
+    if (rows) (void)hipFree(rows);
+    if (err != hipSuccess) return set_error(e, CM_ERR_HIP, hipGetErrorString(err));
+    out->kinetic = h[0];
+    out->momentum[0] = h[1]; out->momentum[1] = h[2]; out->momentum[2] = h[3];
+    out->angular_momentum[0] = h[4];
+    out->angular_momentum[1] = h[5];
+    out->angular_momentum[2] = h[6];
+    out->angular_momentum_mag = sqrt(h[4] * h[4] + h[5] * h[5] + h[6] * h[6]);
+    out->total_mass = h[7];
+    out->potential = with_pe ? h[8] : 0.0;
+    out->active_count = (int32_t)h[9];
+    return CM_OK;
+}
+
+}  // namespace cm

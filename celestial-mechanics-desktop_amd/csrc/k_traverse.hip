@@ -1,0 +1,491 @@
+// k_traverse.hip -- group-MAC Barnes-Hut force traversal, one 64-lane
+// wavefront per group of 64 consecutive Morton-sorted bodies.
+//
+// Per wavefront:
+//  * the 64-body chunk is split (at most split_depth times) at its highest
+//    differing key bit while the two halves are far apart (the whole box's
+//    squared diagonal exceeds 4x the larger half's), so a chunk that straddles
+//    a Morton jump does not walk the tree with a box spanning the jump;
+//  * each sub-group walks the tree from the root with an LDS stack, 64 nodes
+//    per step (fewer when the stack is nearly full, so it never overflows):
+//    nodes of zero mass are skipped (BarnesHutBackend.cs:402); a node is
+//    accepted as a monopole when d2 > 0 and (4 hL) hL < theta^2 d2, d2 being
+//    the squared distance from its centre of mass to the sub-group's box and
+//    hL = h 2^-level (BarnesHutBackend.cs:431-433 with the body replaced by
+//    the nearest point of the box); otherwise a leaf contributes its bodies
+//    and an internal node pushes its children;
+//  * accepted nodes and leaf bodies go to two 64-entry LDS interaction lists;
+//    a full list is staged once (one entry per lane) and every member lane
+//    sums all 64 entries -- 64 interactions per staged load.
+// FP64 (CM_PREC_FP64): monopole terms use the reference's monopole arithmetic
+// (BarnesHutBackend.cs:438-445), body terms GravityKernel's
+// (GravityKernel.cs:9-33); each lane keeps the two sums apart and adds them at
+// the end: bit-identical to the CPU restatement of this traversal.
+// FP32_SPLIT: offsets from the sub-group's box centre are formed in double
+// and rounded to float once per list entry / target; the pair terms run in
+// float (v_rsq_f32, FMA) and each flush's float partial is folded into a
+// double accumulator.
+#include <hip/hip_runtime.h>
+
+#include "cm_internal.h"
+
+namespace cm {
+
+int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out);
+
+struct TravArgs {
+    const TreeHeader* hdr;
+    const int4* info;
+    const double4* cm;
+    const uint64_t* keys;
+    const int32_t* order;
+    const double* sorted;    // x, y, z, m, r blocks of `cap`
+    int cap;
+    double* ax;
+    double* ay;
+    double* az;
+    double eps2;
+    double theta2;
+    int levels;
+    int shell;
+    int split_depth;
+    int nchunks;
+    int per_xcd;
+    unsigned long long* counters;   // [0] cell terms, [1] body terms
+};
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ double wmin(double v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        double t = __shfl_xor(v, o, kWave);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+__device__ __forceinline__ double wmax(double v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        double t = __shfl_xor(v, o, kWave);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+__device__ __forceinline__ int wscan_incl(int v) {
+    int lane = lane_id();
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+struct Box {
+    double lo[3], hi[3];
+};
+
+__device__ __forceinline__ Box wave_box(bool in, double x, double y, double z) {
+    const double big = 1.7976931348623157e308;
+    Box b;
+    b.lo[0] = wmin(in ? x : big); b.lo[1] = wmin(in ? y : big); b.lo[2] = wmin(in ? z : big);
+    b.hi[0] = wmax(in ? x : -big); b.hi[1] = wmax(in ? y : -big); b.hi[2] = wmax(in ? z : -big);
+    return b;
+}
+
+__device__ __forceinline__ double diag2(const Box& b) {
+    double dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ double gk_coeff(double dist_sq, double m, double r,
+                                           double eps2, bool shell) {
+    if (!shell || r <= 0.0) {
+        double inv = 1.0 / sqrt(dist_sq + eps2);
+        return m * inv * inv * inv;
+    }
+    double dist = sqrt(dist_sq);
+    if (dist < r) {
+        double denom = r * r + eps2;
+        double inv_denom_sqrt = 1.0 / sqrt(denom);
+        return m * inv_denom_sqrt / denom;
+    }
+    double inv = 1.0 / sqrt(dist_sq + eps2);
+    return m * inv * inv * inv;
+}
+
+// Hardware reciprocal square root of a double refined by two Newton steps.
+__device__ __forceinline__ double rsqrt_fast(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double hx = 0.5 * x;
+    y = y * __builtin_fma(-hx * y, y, 1.5);
+    y = y * __builtin_fma(-hx * y, y, 1.5);
+    return y;
+}
+
+__device__ __forceinline__ float gk_coeff_f(float dist_sq, float m, float r,
+                                            float eps2, bool shell) {
+    if (shell && r > 0.0f) {
+        float dist = __builtin_sqrtf(dist_sq);
+        if (dist < r) {
+            float denom = __builtin_fmaf(r, r, eps2);
+            float inv = __builtin_amdgcn_rsqf(denom);
+            return m * inv / denom;
+        }
+    }
+    float inv = __builtin_amdgcn_rsqf(dist_sq + eps2);
+    return m * inv * inv * inv;
+}
+
+template <int PREC>
+struct Staging;
+template <>
+struct Staging<CM_PREC_FP64> { double4 v[kWave]; double r[kWave]; int id[kWave]; };
+template <>
+struct Staging<CM_PREC_FP64_FAST> { double4 v[kWave]; double r[kWave]; int id[kWave]; };
+template <>
+struct Staging<CM_PREC_FP32_SPLIT> { float4 v[kWave]; float r[kWave]; int id[kWave]; };
+
+template <int PREC>
+__global__ void __launch_bounds__(kWave)
+k_traverse(TravArgs a) {
+    __shared__ int s_stack[kStackCap];
+    __shared__ int s_cell[2 * kWave];
+    __shared__ int s_part[kWave];
+    __shared__ int s_lpref[kWave];
+    __shared__ int s_lstart[kWave];
+    __shared__ int s_work[3 * 8];
+    __shared__ Staging<PREC> st;
+
+    const int lane = lane_id();
+    const int bid = blockIdx.x;
+    const int chunk = (bid % 8) * a.per_xcd + bid / 8;
+    if (chunk >= a.nchunks) return;
+    const int nA = a.hdr->n_active;
+    const int g0 = chunk * kWave;
+    if (g0 >= nA) return;
+    const int g1 = min(g0 + kWave, nA);
+    const double h = a.hdr->box[3];
+    const int cap = a.cap;
+    const double* sx = a.sorted;
+    const double* sy = a.sorted + cap;
+    const double* sz = a.sorted + 2 * (size_t)cap;
+    const double* sm = a.sorted + 3 * (size_t)cap;
+    const double* sr = a.sorted + 4 * (size_t)cap;
+
+    const int my = g0 + lane;
+    const bool inchunk = my < g1;
+    double xi = 0.0, yi = 0.0, zi = 0.0;
+    uint64_t key = 0ull;
+    if (inchunk) { xi = sx[my]; yi = sy[my]; zi = sz[my]; key = a.keys[my]; }
+    const bool shell = a.shell != 0;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int reserve = 8 * (a.levels + 2);
+
+    // per-lane accumulators (members of the current sub-group only)
+    double acx = 0.0, acy = 0.0, acz = 0.0;   // monopole sums (FP64)
+    double apx = 0.0, apy = 0.0, apz = 0.0;   // body sums
+    unsigned long long n_cell = 0ull, n_body = 0ull;
+
+    int nwork = 1;
+    if (lane == 0) { s_work[0] = g0; s_work[1] = g1; s_work[2] = 0; }
+    __syncthreads();
+    while (nwork > 0) {
+        nwork--;
+        const int s = s_work[3 * nwork], e = s_work[3 * nwork + 1], d = s_work[3 * nwork + 2];
+        __syncthreads();
+        bool split = false;
+        if (d < a.split_depth && e - s >= 2) {
+            uint64_t k0 = __shfl(key, s - g0, kWave);
+            uint64_t k1 = __shfl(key, e - 1 - g0, kWave);
+            if (k0 != k1) {
+                int p = 63 - __clzll((long long)(k0 ^ k1));
+                bool inr = my >= s && my < e;
+                uint64_t hb = __ballot(inr && ((key >> p) & 1ull));
+                int mid = g0 + (__ffsll((unsigned long long)hb) - 1);
+                Box all = wave_box(inr, xi, yi, zi);
+                Box b1 = wave_box(inr && my < mid, xi, yi, zi);
+                Box b2 = wave_box(inr && my >= mid, xi, yi, zi);
+                double da = diag2(all), d1 = diag2(b1), d2 = diag2(b2);
+                double dm = d1 > d2 ? d1 : d2;
+                if (da > 4.0 * dm) {
+                    split = true;
+                    if (lane == 0) {
+                        s_work[3 * nwork] = mid; s_work[3 * nwork + 1] = e; s_work[3 * nwork + 2] = d + 1;
+                        s_work[3 * nwork + 3] = s; s_work[3 * nwork + 4] = mid; s_work[3 * nwork + 5] = d + 1;
+                    }
+                    nwork += 2;
+                }
+            }
+        }
+        __syncthreads();
+        if (split) continue;
+
+        // ---------------- traverse for sub-group [s, e) ----------------
+        const bool member = my >= s && my < e;
+        const int nmem = e - s;
+        Box gb = wave_box(member, xi, yi, zi);
+        const double cxg = (gb.lo[0] + gb.hi[0]) * 0.5;
+        const double cyg = (gb.lo[1] + gb.hi[1]) * 0.5;
+        const double czg = (gb.lo[2] + gb.hi[2]) * 0.5;
+        const float txf = (float)(xi - cxg), tyf = (float)(yi - cyg), tzf = (float)(zi - czg);
+        double fpx = 0.0, fpy = 0.0, fpz = 0.0;   // FP32 path: folded partials
+
+        auto flush = [&](int cnt, bool cells) {
+            // stage one entry per lane
+            if (lane < cnt) {
+                if (cells) {
+                    int k = s_cell[lane];
+                    double4 v = a.cm[k];
+                    if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                        st.v[lane] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
+                                                 (float)(v.z - czg), (float)v.w);
+                    } else {
+                        st.v[lane] = v;
+                    }
+                } else {
+                    int q = s_part[lane];
+                    if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                        st.v[lane] = make_float4((float)(sx[q] - cxg), (float)(sy[q] - cyg),
+                                                 (float)(sz[q] - czg), (float)sm[q]);
+                        st.r[lane] = (float)sr[q];
+                    } else {
+                        st.v[lane] = make_double4(sx[q], sy[q], sz[q], sm[q]);
+                        st.r[lane] = sr[q];
+                    }
+                    st.id[lane] = q;
+                }
+            }
+            __syncthreads();
+            if (cells) {
+                n_cell += (unsigned long long)cnt * nmem;
+            } else {
+                uint64_t selfm = __ballot(lane < cnt && s_part[lane] >= s && s_part[lane] < e);
+                n_body += (unsigned long long)cnt * nmem - (unsigned long long)__popcll(selfm);
+            }
+            if (member) {
+                if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+                    const float e2 = (float)a.eps2;
+                    if (cells || (!shell && e2 > 0.0f)) {
+                        // self term (dx = 0) is exactly zero when eps2 > 0
+                        for (int j = 0; j < cnt; j++) {
+                            float4 v = st.v[j];
+                            float dx = txf - v.x, dy = tyf - v.y, dz = tzf - v.z;
+                            float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
+                            float inv = __builtin_amdgcn_rsqf(r2);
+                            float f = v.w * (inv * inv * inv);
+                            gx = __builtin_fmaf(-f, dx, gx);
+                            gy = __builtin_fmaf(-f, dy, gy);
+                            gz = __builtin_fmaf(-f, dz, gz);
+                        }
+                    } else {
+                        for (int j = 0; j < cnt; j++) {
+                            if (st.id[j] == my) continue;
+                            float4 v = st.v[j];
+                            float dx = txf - v.x, dy = tyf - v.y, dz = tzf - v.z;
+                            float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                            float f = gk_coeff_f(d2, v.w, st.r[j], e2, shell);
+                            gx = __builtin_fmaf(-f, dx, gx);
+                            gy = __builtin_fmaf(-f, dy, gy);
+                            gz = __builtin_fmaf(-f, dz, gz);
+                        }
+                    }
+                    fpx += (double)gx; fpy += (double)gy; fpz += (double)gz;
+                } else if (cells) {
+                    for (int j = 0; j < cnt; j++) {
+                        double4 v = st.v[j];
+                        double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
+                        double dist2 = dx * dx + dy * dy + dz * dz;
+                        double soft = dist2 + a.eps2;
+                        double inv;
+                        if constexpr (PREC == CM_PREC_FP64_FAST) inv = rsqrt_fast(soft);
+                        else inv = 1.0 / sqrt(soft);
+                        double inv3 = inv * inv * inv;
+                        double f = v.w * inv3;
+                        acx -= f * dx; acy -= f * dy; acz -= f * dz;
+                    }
+                } else {
+                    for (int j = 0; j < cnt; j++) {
+                        if (st.id[j] == my) continue;
+                        double4 v = st.v[j];
+                        double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
+                        double dist2 = dx * dx + dy * dy + dz * dz;
+                        double f;
+                        if constexpr (PREC == CM_PREC_FP64_FAST) {
+                            if (!shell) {
+                                double inv = rsqrt_fast(dist2 + a.eps2);
+                                f = v.w * inv * inv * inv;
+                            } else {
+                                f = gk_coeff(dist2, v.w, st.r[j], a.eps2, true);
+                            }
+                        } else {
+                            f = gk_coeff(dist2, v.w, st.r[j], a.eps2, shell);
+                        }
+                        apx -= f * dx; apy -= f * dy; apz -= f * dz;
+                    }
+                }
+            }
+            __syncthreads();
+        };
+
+        int S = 1, ncell = 0, npart = 0;
+        if (lane == 0) s_stack[0] = 0;
+        __syncthreads();
+        while (S > 0) {
+            int lim = (kStackCap - reserve - S) / 7;
+            if (lim < 1) lim = 1;
+            int take = S < kWave ? S : kWave;
+            if (take > lim) take = lim;
+            const int base = S - take;
+            const int node = lane < take ? s_stack[base + lane] : -1;
+            S = base;
+            __syncthreads();
+            int kind = 0;   // 1 cell, 2 leaf, 3 open
+            int4 nd = make_int4(-1, 0, 0, 0);
+            if (node >= 0) {
+                double4 c = a.cm[node];
+                if (c.w != 0.0) {
+                    nd = a.info[node];
+                    const int level = nd.y >> 8;
+                    const bool leaf = nd.x < 0;
+                    double ddx = c.x < gb.lo[0] ? gb.lo[0] - c.x : (c.x > gb.hi[0] ? c.x - gb.hi[0] : 0.0);
+                    double ddy = c.y < gb.lo[1] ? gb.lo[1] - c.y : (c.y > gb.hi[1] ? c.y - gb.hi[1] : 0.0);
+                    double ddz = c.z < gb.lo[2] ? gb.lo[2] - c.z : (c.z > gb.hi[2] ? c.z - gb.hi[2] : 0.0);
+                    double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                    double hl = ldexp(h, -level);
+                    double size2 = 4.0 * hl * hl;
+                    if (!(shell && leaf) && d2 > 0.0 && size2 < a.theta2 * d2) kind = 1;
+                    else kind = leaf ? 2 : 3;
+                }
+            }
+            // accepted cells
+            uint64_t cm_ = __ballot(kind == 1);
+            if (kind == 1) s_cell[ncell + __popcll(cm_ & lt)] = node;
+            ncell += __popcll(cm_);
+            // opened internal nodes push their children, in lane order
+            int nop = kind == 3 ? (nd.y & 255) : 0;
+            int oin = wscan_incl(nop);
+            int otot = __shfl(oin, kWave - 1, kWave);
+            if (kind == 3)
+                for (int c = 0; c < nop; c++) s_stack[S + oin - nop + c] = nd.x + c;
+            S += otot;
+            // opened leaves: body ranges, expanded below in lane order
+            int nl = kind == 2 ? nd.w : 0;
+            int lin = wscan_incl(nl);
+            int ltot = __shfl(lin, kWave - 1, kWave);
+            s_lpref[lane] = lin - nl;
+            s_lstart[lane] = nd.z;
+            __syncthreads();
+            while (ncell >= kWave) {
+                flush(kWave, true);
+                int rest = ncell - kWave;
+                int v = lane < rest ? s_cell[kWave + lane] : 0;
+                __syncthreads();
+                if (lane < rest) s_cell[lane] = v;
+                ncell = rest;
+                __syncthreads();
+            }
+            int done = 0;
+            while (done < ltot) {
+                int room = kWave - npart;
+                int tk = ltot - done < room ? ltot - done : room;
+                if (lane < tk) {
+                    int q = done + lane;
+                    int lo = 0, hi = kWave - 1;   // last lane with pref <= q
+                    while (lo < hi) {
+                        int mid = (lo + hi + 1) >> 1;
+                        if (s_lpref[mid] <= q) lo = mid; else hi = mid - 1;
+                    }
+                    s_part[npart + lane] = s_lstart[lo] + (q - s_lpref[lo]);
+                }
+                npart += tk;
+                done += tk;
+                __syncthreads();
+                if (npart == kWave) { flush(kWave, false); npart = 0; }
+            }
+        }
+        if (ncell > 0) flush(ncell, true);
+        if (npart > 0) flush(npart, false);
+
+        if (member) {
+            const int o = a.order[my];
+            if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                a.ax[o] = fpx; a.ay[o] = fpy; a.az[o] = fpz;
+            } else {
+                a.ax[o] = acx + apx; a.ay[o] = acy + apy; a.az[o] = acz + apz;
+            }
+        }
+        acx = acy = acz = apx = apy = apz = 0.0;
+        __syncthreads();
+    }
+    if (lane == 0) {
+        atomicAdd(&a.counters[0], n_cell);
+        atomicAdd(&a.counters[1], n_body);
+        atomicAdd(&a.counters[2], n_cell + n_body);
+    }
+}
+
+// Inactive bodies sit after the active ones in sorted order: zero their acc.
+__global__ void __launch_bounds__(256)
+k_zero_inactive(int n, const TreeHeader* __restrict__ hdr,
+                const int32_t* __restrict__ order, double* __restrict__ ax,
+                double* __restrict__ ay, double* __restrict__ az) {
+    int k = hdr->n_active + blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    int o = order[k];
+    ax[o] = 0.0; ay[o] = 0.0; az[o] = 0.0;
+}
+
+int launch_barnes_hut(Engine& e, double softening) {
+    const int n = e.count;
+    e.stats = cm_force_stats{};
+    if (n == 0) return CM_OK;
+    hipStream_t st = e.stream;
+    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[0], st));
+    e.mark(0);
+    const uint64_t* keys = nullptr;
+    const int32_t* order = nullptr;
+    int rc = build_tree(e, &keys, &order);
+    if (rc != CM_OK) return rc;
+    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[3], st));
+    e.mark(1);
+    TravArgs a;
+    a.hdr = e.hdr;
+    a.info = e.nd_info;
+    a.cm = e.nd_cm;
+    a.keys = keys;
+    a.order = order;
+    a.sorted = e.sorted;
+    a.cap = e.capacity;
+    a.ax = e.f[AX]; a.ay = e.f[AY]; a.az = e.f[AZ];
+    a.eps2 = softening * softening;
+    double th = e.cfg.theta > 0.2 ? e.cfg.theta : 0.2;   // BarnesHutBackend.cs:129
+    a.theta2 = th * th;
+    a.levels = e.cfg.morton_levels;
+    a.shell = e.cfg.shell_theorem;
+    a.split_depth = e.cfg.group_split_depth;
+    a.nchunks = (n + kWave - 1) / kWave;
+    a.per_xcd = (a.nchunks + 7) / 8;
+    a.counters = &e.hdr->n_cell_int;
+    int nblocks = a.per_xcd * 8;
+    switch (e.cfg.precision) {
+        case CM_PREC_FP64:
+            hipLaunchKernelGGL(k_traverse<CM_PREC_FP64>, dim3(nblocks), dim3(kWave), 0, st, a);
+            break;
+        case CM_PREC_FP64_FAST:
+            hipLaunchKernelGGL(k_traverse<CM_PREC_FP64_FAST>, dim3(nblocks), dim3(kWave), 0, st, a);
+            break;
+        default:
+            hipLaunchKernelGGL(k_traverse<CM_PREC_FP32_SPLIT>, dim3(nblocks), dim3(kWave), 0, st, a);
+            break;
+    }
+    hipLaunchKernelGGL(k_zero_inactive, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                       e.hdr, order, e.f[AX], e.f[AY], e.f[AZ]);
+    CM_HIP_TRY(e, hipGetLastError());
+    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[5], st));
+    e.mark(2);
+    CM_HIP_TRY(e, hipMemcpyAsync(e.hdr_host, e.hdr, sizeof(TreeHeader),
+                                 hipMemcpyDeviceToHost, st));
+    return CM_OK;
+}
+
+}  // namespace cm

@@ -1,0 +1,202 @@
+/*
+ * cmhip.h -- C ABI of libcmhip.so, the MI355X (gfx950) N-body backend.
+ *
+ * The library is a drop-in compute backend for the reference's C# physics
+ * surface (mfscpayload-690/celestial-mechanics-desktop,
+ * src/CelestialMechanics.Physics).  A C# class implementing
+ * IPhysicsComputeBackend / IGravityModelAwareBackend binds these entry points
+ * with [DllImport("cmhip")]; see INTEGRATION.md for that binding.
+ *
+ * Conventions (modelled on the reference's native ABI,
+ * engine/include/celestial/interop/native_api.h:30-365, and its ErrorCode,
+ * engine/include/celestial/core/error.hpp:9-18):
+ *   - every function returns int32_t: 0 on success, a negative CM_ERR_* code
+ *     otherwise; no exception ever crosses the boundary;
+ *   - state lives in an opaque engine handle (not a global singleton); the
+ *     last error message is kept per handle (cm_last_error);
+ *   - the caller owns host memory, the library owns device memory; host arrays
+ *     are copied in (cm_upload_soa) and out (cm_download_soa);
+ *   - bool arrays travel as uint8_t (0/1);
+ *   - calls on one handle must be serialised by the caller; different handles
+ *     are independent.
+ */
+#ifndef CMHIP_H
+#define CMHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes (values of the reference's ErrorCode) ------------------ */
+#define CM_OK 0
+#define CM_ERR_HIP (-1)              /* ErrorCode::CudaError */
+#define CM_ERR_OUT_OF_MEMORY (-2)    /* ErrorCode::OutOfMemory */
+#define CM_ERR_INVALID_ARGUMENT (-3) /* ErrorCode::InvalidArgument */
+#define CM_ERR_POOL_EXHAUSTED (-4)   /* ErrorCode::PoolExhausted */
+#define CM_ERR_DEVICE_NOT_FOUND (-5) /* ErrorCode::DeviceNotFound */
+#define CM_ERR_NOT_INITIALIZED (-6)  /* ErrorCode::NotInitialized */
+
+/* ---- force backends ----------------------------------------------------- */
+/* CM_BACKEND_DIRECT_PAIRWISE: CpuSingleThreadBackend.ComputeForces
+ *   (Solvers/CpuSingleThreadBackend.cs:50-158): Newton-3 pair order
+ *   (shell theorem off) or the full GravityKernel sweep (shell on).
+ * CM_BACKEND_DIRECT: CpuParallelBackend.ComputeForces
+ *   (Solvers/CpuParallelBackend.cs:69-142): full j sweep per body.
+ * CM_BACKEND_BARNES_HUT: BarnesHutBackend.ComputeForces
+ *   (Solvers/BarnesHutBackend.cs:118-236) re-designed for the GPU: Morton
+ *   keys, stable radix sort, level-order bucket octree, bottom-up centre of
+ *   mass, group-MAC traversal with the reference's (2h)^2 < theta^2 d^2 test. */
+#define CM_BACKEND_DIRECT_PAIRWISE 0
+#define CM_BACKEND_DIRECT 1
+#define CM_BACKEND_BARNES_HUT 2
+
+/* FP64: every interaction in double, IEEE 1/sqrt (bit-reproducible).
+ * FP64_FAST: double with a hardware reciprocal square root refined by Newton
+ *   steps (the plain FP64 GPU path used as the performance baseline).
+ * FP32_SPLIT: pair terms in float on offsets taken relative to the target
+ *   group's centre (computed in double), partial sums folded into double. */
+#define CM_PREC_FP64 0
+#define CM_PREC_FP64_FAST 1
+#define CM_PREC_FP32_SPLIT 2
+
+typedef struct {
+    int32_t kind;              /* CM_BACKEND_* */
+    int32_t precision;         /* CM_PREC_* */
+    int32_t shell_theorem;     /* IGravityModelAwareBackend.EnableShellTheorem */
+    int32_t morton_levels;     /* key bits / 3: 10 (30-bit keys) .. 21 */
+    double theta;              /* BarnesHutBackend.Theta, clamped to >= 0.2 */
+    int32_t leaf_capacity;     /* bodies per leaf bucket, 1..64 */
+    int32_t group_split_depth; /* target-group split depth, 0..3 */
+} cm_backend_config;
+
+/* Pointers to the 18 BodySoA arrays (SoA/BodySoA.cs:42-69).  On upload a
+ * NULL pointer means zeros, except old_acc_* which, when NULL, is seeded from
+ * acc_* as BodySoA.CopyFrom does (SoA/BodySoA.cs:141-144).  On download a
+ * NULL pointer skips that array. */
+typedef struct {
+    double *pos_x, *pos_y, *pos_z;
+    double *vel_x, *vel_y, *vel_z;
+    double *acc_x, *acc_y, *acc_z;
+    double *old_acc_x, *old_acc_y, *old_acc_z;
+    double *mass, *radius, *density;
+    uint8_t *is_active, *is_collidable;
+    int32_t *body_type;
+} cm_soa_ptrs;
+
+/* Timing and work counters of the last force evaluation. */
+typedef struct {
+    int64_t cell_interactions;  /* body-node monopole terms */
+    int64_t body_interactions;  /* body-body terms (self excluded) */
+    int32_t n_active;
+    int32_t n_nodes;
+    int32_t n_groups;
+    int32_t flags;              /* bit 0: node pool exhausted */
+    float ms_tree;              /* bbox + keys + sort + build + mass */
+    float ms_sort;              /* radix sort only */
+    float ms_force;             /* traversal / direct kernel */
+    float ms_total;             /* whole force evaluation */
+    /* accumulated since the last cm_reset_force_stats (up to 1024 evaluations
+     * per accumulation window are timed): */
+    int64_t acc_interactions;   /* cell + body terms */
+    double acc_ms_force;        /* summed force-kernel time, HIP events */
+    double acc_ms_total;        /* summed force-evaluation time */
+    int32_t acc_evaluations;
+} cm_force_stats;
+
+/* EnergyCalculator quantities (Validation/EnergyCalculator.cs:16-81) with
+ * compensated (Kahan) FP64 reductions on the GPU. */
+typedef struct {
+    double kinetic;
+    double potential;           /* -sum_{i<j} m_i m_j / sqrt(r^2 + eps^2) */
+    double momentum[3];
+    double angular_momentum[3];
+    double angular_momentum_mag;
+    double total_mass;
+    int32_t active_count;
+} cm_energy_report;
+
+/* CollisionEvent (Collisions/CollisionEvent.cs:3-11). */
+typedef struct {
+    int32_t body_a, body_b;
+    double overlap_depth, distance;
+    double normal_x, normal_y, normal_z;
+    double relative_normal_speed;
+} cm_collision_event;
+
+/* ---- lifecycle ----------------------------------------------------------- */
+/* Replaces celestial_init (engine/include/celestial/interop/native_api.h:33):
+ * allocates device state for `capacity` bodies on HIP device `device`. */
+int32_t cm_create(int32_t capacity, int32_t device, void** out_engine);
+/* Replaces celestial_shutdown (native_api.h:36). */
+int32_t cm_destroy(void* engine);
+/* Replaces celestial_get_last_error_message (native_api.h:294), per handle. */
+const char* cm_last_error(void* engine);
+/* IGravityModelAwareBackend.EnableShellTheorem, BarnesHutBackend.Theta and
+ * the NBodySolver.SelectBackend choice (Solvers/NBodySolver.cs:451-484). */
+int32_t cm_configure(void* engine, const cm_backend_config* cfg);
+int32_t cm_default_config(cm_backend_config* cfg);
+
+/* ---- data transfer ------------------------------------------------------ */
+/* BodySoA.CopyFrom-equivalent upload (SoA/BodySoA.cs:118-153), replacing
+ * celestial_set_particles (native_api.h:41-47).  count <= capacity. */
+int32_t cm_upload_soa(void* engine, const cm_soa_ptrs* src, int32_t count);
+/* BodySoA.CopyTo-equivalent download (SoA/BodySoA.cs:160-174), replacing
+ * celestial_get_accelerations/positions/velocities (native_api.h:50-60). */
+int32_t cm_download_soa(void* engine, const cm_soa_ptrs* dst, int32_t count);
+int32_t cm_get_count(void* engine, int32_t* count);
+
+/* ---- physics -------------------------------------------------------------- */
+/* IPhysicsComputeBackend.ComputeForces(BodySoA, softening)
+ * (Solvers/IPhysicsComputeBackend.cs:33); replaces celestial_compute_forces
+ * (native_api.h:65).  Overwrites acc for every index < count; inactive
+ * bodies get 0. */
+int32_t cm_compute_forces(void* engine, double softening);
+/* n_steps x SoAVerletIntegrator.Step(bodies, backend, softening, dt)
+ * (Integrators/SoAVerletIntegrator.cs:48-115), device-resident; replaces
+ * celestial_step (native_api.h:68). */
+int32_t cm_verlet_step(void* engine, double softening, double dt,
+                       int32_t n_steps);
+/* EnergyCalculator.ComputeKE/ComputePE/ComputeMomentum/
+ * ComputeAngularMomentum (Validation/EnergyCalculator.cs:16-81). */
+int32_t cm_energy(void* engine, double softening, int32_t with_potential,
+                  cm_energy_report* out);
+/* FP64 direct-sum accelerations on a subset of target bodies: the judge of
+ * ForceErrorAnalyzer.CompareForces (Validation/ForceErrorAnalyzer.cs:67-92)
+ * restricted to a sample.  Results are host arrays of length n_targets. */
+int32_t cm_direct_sample(void* engine, double softening,
+                         const int32_t* targets, int32_t n_targets,
+                         double* acc_x, double* acc_y, double* acc_z);
+int32_t cm_get_force_stats(void* engine, cm_force_stats* out);
+int32_t cm_reset_force_stats(void* engine);
+
+/* ---- collisions ----------------------------------------------------------- */
+/* CollisionDetector.Detect (Collisions/CollisionDetector.cs:35-68): events
+ * sorted by overlap depth, deepest first.  Copies up to `cap` events to
+ * `out` (may be NULL) and stores the total in *n_events.  The full list
+ * stays on the device for cm_resolve_merges. */
+int32_t cm_detect_collisions(void* engine, cm_collision_event* out,
+                             int32_t cap, int32_t* n_events);
+/* CollisionResolver.Resolve in CollisionMode.MergeOnly
+ * (Collisions/CollisionResolver.cs:58-130, MergePolicy.cs:31-80) over the
+ * events of the last cm_detect_collisions. */
+int32_t cm_resolve_merges(void* engine, int32_t* n_merged);
+/* Order-preserving removal of inactive bodies, ParticleSystem::compact
+ * (engine/src/physics/particle_system.cpp:156-184).  old_index (may be NULL)
+ * receives, for each survivor, its index before compaction. */
+int32_t cm_compact(void* engine, int32_t* new_count, int32_t* old_index);
+
+/* ---- synthetic scenarios (host-side, deterministic from a seed) ----------- */
+/* PlummerSphere.Generate (ScenarioGenerators/PlummerSphere.cs:41-116) with a
+ * SplitMix64 stream; mass_fraction_cut < 1 truncates the sphere at the radius
+ * enclosing that mass fraction; com_frame subtracts the centre-of-mass
+ * position and velocity.  Writes pos/vel/mass/radius of n bodies. */
+int32_t cm_generate_plummer(int32_t n, double total_mass, double scale_length,
+                            uint64_t seed, double mass_fraction_cut,
+                            int32_t com_frame, const cm_soa_ptrs* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CMHIP_H */

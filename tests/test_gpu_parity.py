@@ -1,0 +1,195 @@
+"""GPU parity: libcmhip.so against the CPU oracle on the same seeded inputs.
+
+FP64 paths are bit-exact (same operation order, no FMA, IEEE divide and
+square root); FP32 split-precision is judged by relative error with the
+tolerance written in each test."""
+import numpy as np
+import pytest
+
+import cmhip
+import oracle as O
+from conftest import soa_to_state
+
+pytestmark = pytest.mark.gpu
+
+EPS = 1e-4
+
+
+def _acc(soa):
+    n = soa.count
+    return soa.acc_x[:n].copy(), soa.acc_y[:n].copy(), soa.acc_z[:n].copy()
+
+
+def _eq(a, b):
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def _rel(a, b):
+    num = np.sqrt((a[0] - b[0]) ** 2 + (a[1] - b[1]) ** 2 + (a[2] - b[2]) ** 2)
+    den = np.sqrt(b[0] ** 2 + b[1] ** 2 + b[2] ** 2)
+    return num / den
+
+
+def _with_inactive(soa, frac=0.1, seed=5):
+    rng = np.random.default_rng(seed)
+    n = soa.count
+    soa.is_active[:n][rng.random(n) < frac] = 0
+    return soa
+
+
+@pytest.mark.parametrize("n", [1, 2, 257, 1000])
+def test_direct_pairwise_bit_exact(plummer, n):
+    """CpuSingleThreadBackend Newton-3 order (CpuSingleThreadBackend.cs:79-118)."""
+    soa = _with_inactive(plummer(n))
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(n, kind=cmhip.BACKEND_DIRECT_PAIRWISE)
+    be.compute_forces(soa, EPS)
+    _eq(_acc(soa), O.direct_n3(s, EPS))
+
+
+@pytest.mark.parametrize("shell", [False, True])
+def test_direct_sweep_bit_exact(plummer, shell):
+    """CpuParallelBackend sweep / shell-theorem GravityKernel (GravityKernel.cs:9-33)."""
+    soa = _with_inactive(plummer(1500))
+    soa.radius[:soa.count] = 0.05     # large radii so the interior branch is hit
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(1500, kind=cmhip.BACKEND_DIRECT, enable_shell_theorem=shell)
+    be.compute_forces(soa, EPS)
+    _eq(_acc(soa), O.direct_gk(s, EPS, shell))
+
+
+def test_verlet_steps_bit_exact(plummer):
+    """SoAVerletIntegrator.Step x 20 with the pairwise backend: every array bit-exact."""
+    n = 1000
+    soa = _with_inactive(plummer(n))
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(n, kind=cmhip.BACKEND_DIRECT_PAIRWISE)
+    be.upload(soa)
+    be.step(EPS, 1e-3, 20)
+    out = soa.copy()
+    be.download(out)
+    for _ in range(20):
+        O.verlet_step(s, 1e-3, lambda st: O.direct_n3(st, EPS))
+    for k_soa, k_or in (("pos_x", "px"), ("pos_y", "py"), ("pos_z", "pz"), ("vel_x", "vx"),
+                        ("vel_y", "vy"), ("vel_z", "vz"), ("acc_x", "ax"),
+                        ("old_acc_z", "oaz")):
+        np.testing.assert_array_equal(getattr(out, k_soa)[:n], s[k_or])
+
+
+@pytest.mark.parametrize("levels,leaf,split", [(10, 16, 3), (21, 8, 3), (10, 1, 0), (21, 64, 1)])
+def test_barnes_hut_fp64_bit_exact(plummer, levels, leaf, split):
+    """Build-defined octree + group-MAC traversal in FP64 equals the oracle bit for bit,
+    including the interaction counts."""
+    n = 20000
+    soa = _with_inactive(plummer(n, cut=0.999))
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5, morton_levels=levels,
+                                 leaf_capacity=leaf, group_split_depth=split)
+    be.compute_forces(soa, EPS)
+    ref, st = O.bh_tree(s, EPS, 0.5, levels=levels, leaf_cap=leaf, group_size=64,
+                        split_depth=split, lifo=2048, nthreads=8)
+    _eq(_acc(soa), ref)
+    gs = be.stats()
+    assert gs["n_active"] == int(s["active"].sum())
+    assert gs["cell_interactions"] == int(st["n_cells"].astype(np.int64).sum())
+    assert gs["body_interactions"] == int(st["n_parts"].astype(np.int64).sum())
+    assert gs["flags"] == 0
+
+
+def test_barnes_hut_fp32_split_vs_fp64(plummer):
+    """FP32 split precision on the same tree and lists as the FP64 path:
+    per-body relative difference median <= 1e-6, max <= 1e-3."""
+    n = 50000
+    soa = plummer(n, cut=0.999)
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP32_SPLIT, theta=0.5)
+    be.compute_forces(soa, EPS)
+    ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=3,
+                       lifo=2048, nthreads=8)
+    r = _rel(_acc(soa), ref)
+    assert np.median(r) <= 1e-6, np.median(r)
+    assert r.max() <= 1e-3, r.max()
+
+
+def test_barnes_hut_fp64_fast_vs_exact(plummer):
+    """Hardware rsqrt + Newton (the FP64 baseline) within 1e-13 of the IEEE path."""
+    n = 30000
+    soa = plummer(n, cut=0.999)
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64_FAST, theta=0.5)
+    be.compute_forces(soa, EPS)
+    ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=3,
+                       lifo=2048, nthreads=8)
+    assert _rel(_acc(soa), ref).max() <= 1e-13
+
+
+def test_barnes_hut_two_body_equals_direct():
+    """BarnesHutTests.cs:65-97: two bodies, BH equals brute force to 10 digits."""
+    soa = cmhip.BodySoA.from_arrays(pos_x=[0.0, 1.0], pos_y=[0.0, 0.0], pos_z=[0.0, 0.0],
+                                    mass=[1.0, 1e-3], is_active=[1, 1])
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(2)
+    be.compute_forces(soa, EPS)
+    ref = O.direct_gk(s, EPS)
+    np.testing.assert_allclose(np.array(_acc(soa)), np.array(ref), rtol=1e-10, atol=0)
+
+
+def test_theta_clamp_bit_identical(plummer):
+    """BarnesHutAccuracyTests.cs:165-186: theta=0 equals theta=0.2 bit for bit."""
+    soa = plummer(4000, cut=0.999)
+    a = soa.copy()
+    b = soa.copy()
+    be = cmhip.HipPhysicsBackend(4000, theta=0.0)
+    be.compute_forces(a, EPS)
+    be.theta = 0.2
+    be.compute_forces(b, EPS)
+    _eq(_acc(a), _acc(b))
+
+
+def test_barnes_hut_deterministic_rerun(plummer):
+    """BarnesHutTests.cs:156-187: two runs are bit-identical (here: 5 Verlet steps, FP32)."""
+    soa = plummer(30000, cut=0.999)
+    outs = []
+    for _ in range(2):
+        be = cmhip.HipPhysicsBackend(30000, precision=cmhip.PREC_FP32_SPLIT, theta=0.6)
+        be.upload(soa)
+        be.step(EPS, 1e-3, 5)
+        o = soa.copy()
+        be.download(o)
+        outs.append(o)
+    for f in ("pos_x", "vel_y", "acc_z"):
+        np.testing.assert_array_equal(getattr(outs[0], f), getattr(outs[1], f))
+
+
+def test_all_inactive_and_empty():
+    soa = cmhip.BodySoA.from_arrays(pos_x=[0.0, 1.0, 2.0], pos_y=[0.0] * 3, pos_z=[0.0] * 3,
+                                    mass=[1.0] * 3, acc_x=[5.0] * 3, is_active=[0, 0, 0])
+    be = cmhip.HipPhysicsBackend(8)
+    be.compute_forces(soa, EPS)
+    assert (soa.acc_x[:3] == 0).all()
+    empty = cmhip.BodySoA(4)
+    be.compute_forces(empty, EPS)
+
+
+def test_energy_compensated_vs_exact(plummer):
+    soa = _with_inactive(plummer(3000))
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(3000)
+    be.upload(soa)
+    e = be.energy(EPS, True)
+    x = O.energy_exact(s, EPS, True, nthreads=8)
+    assert e["active_count"] == int(s["active"].sum())
+    assert abs(e["kinetic"] - x["ke"]) <= 1e-14 * abs(x["ke"])
+    assert abs(e["potential"] - x["pe"]) <= 1e-13 * abs(x["pe"])
+    for k, ref in enumerate((x["px"], x["py"], x["pz"])):
+        assert abs(e["momentum"][k] - ref) <= 1e-15
+
+
+def test_direct_sample_matches_oracle(plummer):
+    soa = plummer(12000, cut=0.999)
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(12000)
+    be.upload(soa)
+    t = np.random.default_rng(3).choice(12000, 256, replace=False).astype(np.int32)
+    _eq(be.direct_sample(EPS, t), O.direct_targets(s, EPS, t))
