@@ -38,7 +38,7 @@ def log(msg):
 def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample):
     n = soa.count
     be = cmhip.HipPhysicsBackend(n, precision=precision, theta=theta, morton_levels=10,
-                                 leaf_capacity=16, group_split_depth=3)
+                                 leaf_capacity=16, group_size=64)
     be.upload(soa)
     be.step(eps, dt, warmup)
     be.reset_stats()

@@ -50,7 +50,12 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_get_count", "cm_compute_forces", "cm_verlet_step", "cm_energy",
            "cm_direct_sample", "cm_get_force_stats", "cm_detect_collisions",
            "cm_resolve_merges", "cm_compact", "cm_generate_plummer",
-           "cm_reset_force_stats")
+           "cm_reset_force_stats", "cm_generate_disk")
+
+# 3 g/cm^3 in Msun / AU^3 (AU = 1.496e11 m, Msun = 1.989e30 kg), and the same
+# density with radii inflated 100x (density / 100^3)
+ROCK_DENSITY_SIM = 3000.0 * 1.496e11 ** 3 / 1.989e30
+ROCK_DENSITY_INFLATED = ROCK_DENSITY_SIM / 100.0 ** 3
 
 
 class CmError(RuntimeError):
@@ -63,7 +68,7 @@ class BackendConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("precision", C.c_int32),
                 ("shell_theorem", C.c_int32), ("morton_levels", C.c_int32),
                 ("theta", C.c_double), ("leaf_capacity", C.c_int32),
-                ("group_split_depth", C.c_int32)]
+                ("group_size", C.c_int32)]
 
 
 class SoAPtrs(C.Structure):
@@ -129,6 +134,8 @@ def load_library():
         "cm_compact": ([vp, C.POINTER(i32), vp], i32),
         "cm_generate_plummer": ([i32, f64, f64, C.c_uint64, f64, i32,
                                  C.POINTER(SoAPtrs)], i32),
+        "cm_generate_disk": ([i32, C.c_uint64, f64, f64, f64, f64, f64, f64, f64, f64,
+                              C.POINTER(SoAPtrs)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -203,6 +210,20 @@ def generate_plummer(count, total_mass=1.0, scale_length=1.0, seed=42,
     return soa
 
 
+def generate_disk(count, seed=42, star_radius=0.005, a_min=0.5, a_max=5.0, sigma_e=0.02,
+                  sigma_i=0.01, m_min=1e-9, m_max=1e-6, density=ROCK_DENSITY_INFLATED,
+                  capacity=None):
+    """Cold thin disk of count-1 bodies around a 1 Msun star (see cm_generate_disk)."""
+    soa = BodySoA(capacity or max(count, 1))
+    rc = load_library().cm_generate_disk(count, C.c_uint64(seed), star_radius, a_min, a_max,
+                                         sigma_e, sigma_i, m_min, m_max, density,
+                                         C.byref(soa.ptrs()))
+    if rc != CM_OK:
+        raise CmError(rc, "cm_generate_disk")
+    soa.count = count
+    return soa
+
+
 class HipPhysicsBackend:
     """IPhysicsComputeBackend + IGravityModelAwareBackend on the GPU.
 
@@ -213,7 +234,7 @@ class HipPhysicsBackend:
 
     def __init__(self, capacity, kind=BACKEND_BARNES_HUT, precision=PREC_FP64,
                  theta=0.5, morton_levels=10, leaf_capacity=16,
-                 group_split_depth=3, enable_shell_theorem=False, device=0):
+                 group_size=64, enable_shell_theorem=False, device=0):
         L = load_library()
         h = C.c_void_p()
         rc = L.cm_create(int(capacity), int(device), C.byref(h))
@@ -229,7 +250,7 @@ class HipPhysicsBackend:
         self.config.theta = theta
         self.config.morton_levels = morton_levels
         self.config.leaf_capacity = leaf_capacity
-        self.config.group_split_depth = group_split_depth
+        self.config.group_size = group_size
         self.config.shell_theorem = int(enable_shell_theorem)
         self._apply()
 
@@ -365,3 +386,28 @@ class SoAVerletIntegrator:
         backend.upload(bodies)
         backend.step(softening, dt, 1)
         backend.download(bodies)
+
+
+class CollisionDetector:
+    """CollisionDetector.Detect (Collisions/CollisionDetector.cs:35-68) on the
+    device state of a HipPhysicsBackend: returns the events (a structured array
+    with the CollisionEvent fields) sorted deepest overlap first."""
+
+    def detect(self, backend, cap=1 << 16):
+        events, total = backend.detect_collisions(cap)
+        if total > cap:
+            events, total = backend.detect_collisions(total)
+        return events
+
+
+class CollisionResolver:
+    """CollisionResolver.Resolve in CollisionMode.MergeOnly
+    (Collisions/CollisionResolver.cs:58-130) over the events of the last
+    detection on the same backend; last_collision_count mirrors
+    LastCollisionCount."""
+
+    last_collision_count = 0
+
+    def resolve(self, backend):
+        self.last_collision_count = backend.resolve_merges()
+        return self.last_collision_count

@@ -38,6 +38,8 @@ struct TreeHeader {
     unsigned long long n_cell_int;
     unsigned long long n_body_int;
     unsigned long long acc_int;   // cell + body terms since the last stats reset
+    int32_t n_groups;             // target groups formed for the traversal
+    uint32_t work_next;           // next group index handed to a wavefront
 };
 
 constexpr int kWindow = 1024;     // force evaluations timed per host sync
@@ -71,7 +73,9 @@ struct Engine {
     int4* nd_info = nullptr;           // first, nchild | level << 8, start, count
     double4* nd_cm = nullptr;          // com x, y, z, mass
     double* nd_sum = nullptr;          // 3 x node_cap: sum m*x, m*y, m*z
+    double* nd_box = nullptr;          // 6 x node_cap: body bounding box lo, hi
     unsigned long long* lookback = nullptr;  // (levels + 1) x build tiles
+    int2* groups = nullptr;            // target groups: sorted body range [x, y)
 
     // collisions / compaction scratch
     void* coll = nullptr;
@@ -110,6 +114,10 @@ int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne)
 int collide_resolve(Engine& e, int32_t* merged);
 int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index);
 void collide_free(Engine& e);
+
+void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
+                      uint64_t** k, int32_t** v, uint64_t** kt, int32_t** vt,
+                      uint32_t* hist);
 
 int set_error(Engine& e, int code, const std::string& msg);
 int finalize_stats(Engine& e);   // syncs the stream, fills e.stats

@@ -150,7 +150,8 @@ static void free_all(Engine& e) {
     for (int k = 0; k < NF64; k++) if (e.f[k]) (void)hipFree(e.f[k]);
     void* ptrs[] = {e.act, e.col, e.type, e.keys_a, e.keys_b, e.vals_a,
                     e.vals_b, e.hist, e.sorted, e.bbox_part, e.bbox_cnt, e.hdr,
-                    e.nd_info, e.nd_cm, e.nd_sum, e.lookback, e.red, e.sample};
+                    e.nd_info, e.nd_cm, e.nd_sum, e.lookback, e.red, e.sample,
+                    e.groups, e.nd_box};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     if (e.hdr_host) (void)hipHostFree(e.hdr_host);
     collide_free(e);
@@ -178,7 +179,7 @@ int32_t cm_default_config(cm_backend_config* cfg) {
     cfg->morton_levels = 10;
     cfg->theta = 0.5;   // BarnesHutBackend.Theta default (BarnesHutBackend.cs:75)
     cfg->leaf_capacity = 16;
-    cfg->group_split_depth = 3;
+    cfg->group_size = 64;
     return CM_OK;
 }
 
@@ -217,7 +218,7 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     alloc((void**)&e->vals_a, n * 4);
     alloc((void**)&e->vals_b, n * 4);
     size_t tiles = (n + cm::kSortTile - 1) / cm::kSortTile;
-    alloc((void**)&e->hist, 256 * tiles * 4 + 256);
+    alloc((void**)&e->hist, 256 * (tiles + 1) * 4);
     alloc((void**)&e->sorted, 5 * n * 8);
     alloc((void**)&e->bbox_part, 6 * 1024 * 8);
     alloc((void**)&e->bbox_cnt, 1024 * 4);
@@ -226,8 +227,10 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     alloc((void**)&e->nd_info, (size_t)e->node_cap * sizeof(int4));
     alloc((void**)&e->nd_cm, (size_t)e->node_cap * sizeof(double4));
     alloc((void**)&e->nd_sum, (size_t)e->node_cap * 3 * 8);
+    alloc((void**)&e->nd_box, (size_t)e->node_cap * 6 * 8);
     size_t btiles = ((size_t)e->node_cap + cm::kBuildTile - 1) / cm::kBuildTile;
     alloc((void**)&e->lookback, (cm::kMaxLevels + 1) * btiles * 8);
+    alloc((void**)&e->groups, n * sizeof(int2));
     alloc((void**)&e->red, 64 * 1024 * 8);
     alloc((void**)&e->sample, 8);
     if (ok && hipHostMalloc((void**)&e->hdr_host, sizeof(cm::TreeHeader)) != hipSuccess)
@@ -277,9 +280,9 @@ int32_t cm_configure(void* h, const cm_backend_config* cfg) {
     if (cfg->leaf_capacity < 1 || cfg->leaf_capacity > 64)
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT,
                              "leaf_capacity must be in 1..64");
-    if (cfg->group_split_depth < 0 || cfg->group_split_depth > 3)
+    if (cfg->group_size < 1 || cfg->group_size > 64)
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT,
-                             "group_split_depth must be in 0..3");
+                             "group_size must be in 1..64");
     if (!(cfg->theta >= 0.0) || !(cfg->theta < 1e6))
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "theta must be finite, >= 0");
     e->cfg = *cfg;

@@ -1,31 +1,37 @@
-// k_traverse.hip -- group-MAC Barnes-Hut force traversal, one 64-lane
-// wavefront per group of 64 consecutive Morton-sorted bodies.
+// k_traverse.hip -- target-group formation and the group-MAC Barnes-Hut
+// traversal, one 64-lane wavefront per target group.
 //
-// Per wavefront:
-//  * the 64-body chunk is split (at most split_depth times) at its highest
-//    differing key bit while the two halves are far apart (the whole box's
-//    squared diagonal exceeds 4x the larger half's), so a chunk that straddles
-//    a Morton jump does not walk the tree with a box spanning the jump;
-//  * each sub-group walks the tree from the root with an LDS stack, 64 nodes
-//    per step (fewer when the stack is nearly full, so it never overflows):
-//    nodes of zero mass are skipped (BarnesHutBackend.cs:402); a node is
-//    accepted as a monopole when d2 > 0 and (4 hL) hL < theta^2 d2, d2 being
-//    the squared distance from its centre of mass to the sub-group's box and
-//    hL = h 2^-level (BarnesHutBackend.cs:431-433 with the body replaced by
-//    the nearest point of the box); otherwise a leaf contributes its bodies
-//    and an internal node pushes its children;
-//  * accepted nodes and leaf bodies go to two 64-entry LDS interaction lists;
-//    a full list is staged once (one entry per lane) and every member lane
-//    sums all 64 entries -- 64 interactions per staged load.
+// Target groups (k_groups) follow the octree: for every node holding more
+// than group_size bodies, runs of consecutive children that each hold at most
+// group_size bodies are packed, in child order, into groups while a group
+// stays within group_size and stays compact (it is not packed further when
+// the squared diagonal of its box would exceed 4x the larger part's, or its
+// padded volume 4x the parts' summed volume).  A group is a contiguous range
+// of Morton-sorted bodies inside one parent cell, so its box never spans a
+// Morton jump into a distant, denser region.
+//
+// Traversal (k_traverse): persistent wavefronts take groups from a device
+// counter.  Each group walks the tree from the root with an LDS stack,
+// 64 nodes per step (fewer when the stack is nearly full, so it never
+// overflows): nodes of zero mass are skipped (BarnesHutBackend.cs:402); a node
+// is accepted as a monopole when d2 > 0 and (4 hL) hL < theta^2 d2, d2 being
+// the squared distance from its centre of mass to the group's box and
+// hL = h 2^-level (BarnesHutBackend.cs:431-433 with the body replaced by the
+// nearest point of the box); otherwise a leaf contributes its bodies and an
+// internal node pushes its children.  Accepted nodes and leaf bodies go to two
+// 64-entry LDS interaction lists; a full list is staged once (one entry per
+// lane) and every member lane sums all its entries.
 // FP64 (CM_PREC_FP64): monopole terms use the reference's monopole arithmetic
 // (BarnesHutBackend.cs:438-445), body terms GravityKernel's
 // (GravityKernel.cs:9-33); each lane keeps the two sums apart and adds them at
 // the end: bit-identical to the CPU restatement of this traversal.
-// FP32_SPLIT: offsets from the sub-group's box centre are formed in double
-// and rounded to float once per list entry / target; the pair terms run in
-// float (v_rsq_f32, FMA) and each flush's float partial is folded into a
-// double accumulator.
+// FP64_FAST: the same with a hardware reciprocal square root + Newton steps.
+// FP32_SPLIT: offsets from the group's box centre are formed in double and
+// rounded to float once per list entry / target; the pair terms run in float
+// (v_rsq_f32, FMA) and each flush's float partial is folded into double.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "cm_internal.h"
 
@@ -33,12 +39,96 @@ namespace cm {
 
 int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out);
 
+// ------------------------------------------------------------ grouping ----
+__device__ __forceinline__ double diag2_of(const double* lo, const double* hi) {
+    double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+__device__ __forceinline__ double vol_of(const double* lo, const double* hi, double pad) {
+    return ((hi[0] - lo[0]) + pad) * ((hi[1] - lo[1]) + pad) * ((hi[2] - lo[2]) + pad);
+}
+
+__device__ __forceinline__ void emit_group(TreeHeader* hdr, int2* groups, int s, int e) {
+    int idx = atomicAdd(&hdr->n_groups, 1);
+    groups[idx] = make_int2(s, e);
+}
+
+// box_split with the three boxes given directly: the bin [s, mid) as lo1/hi1,
+// the next child [mid, e) as lo2/hi2; the whole box is their union.  Node
+// boxes are exact min/max over their bodies, so this equals box_split on the
+// bodies.
+__device__ bool box_split_nodes(const double* lo1, const double* hi1,
+                                const double* lo2, const double* hi2) {
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) {
+        lo[k] = lo1[k] < lo2[k] ? lo1[k] : lo2[k];
+        hi[k] = hi1[k] > hi2[k] ? hi1[k] : hi2[k];
+    }
+    double d = diag2_of(lo, hi), d1 = diag2_of(lo1, hi1), d2 = diag2_of(lo2, hi2);
+    double dm = d1 > d2 ? d1 : d2;
+    double pad = 1e-3 * ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
+    double v = vol_of(lo, hi, pad), v1 = vol_of(lo1, hi1, pad), v2 = vol_of(lo2, hi2, pad);
+    return d > 4.0 * dm || v > 4.0 * (v1 + v2);
+}
+
+__global__ void __launch_bounds__(256)
+k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
+         const double* __restrict__ nbox, int gsize, int2* __restrict__ groups) {
+    const int nn = hdr->n_nodes;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < nn; k += gridDim.x * 256) {
+        int4 nd = info[k];
+        if (k == 0 && nd.w > 0 && nd.w <= gsize) emit_group(hdr, groups, 0, nd.w);
+        if (nd.w <= gsize) continue;
+        if (nd.x < 0) {
+            // a deepest-level leaf holding more than gsize bodies: consecutive
+            // slices of gsize
+            for (int q = nd.z; q < nd.z + nd.w; q += gsize)
+                emit_group(hdr, groups, q, min(q + gsize, nd.z + nd.w));
+            continue;
+        }
+        int nch = nd.y & 255;
+        int bs = -1, be = -1;
+        double blo[3], bhi[3];   // box of the open bin
+        for (int c = nd.x; c < nd.x + nch; c++) {
+            int4 cn = info[c];
+            int cs = cn.z, ce = cn.z + cn.w;
+            const double* cb = nbox + 6 * (size_t)c;
+            bool close = false, open = false;
+            if (cn.w > gsize) {
+                close = true;
+            } else if (bs < 0) {
+                open = true;
+            } else if (ce - bs <= gsize && !box_split_nodes(blo, bhi, cb, cb + 3)) {
+                be = ce;
+                for (int q = 0; q < 3; q++) {
+                    blo[q] = blo[q] < cb[q] ? blo[q] : cb[q];
+                    bhi[q] = bhi[q] > cb[3 + q] ? bhi[q] : cb[3 + q];
+                }
+            } else {
+                close = true;
+                open = true;
+            }
+            if (close && bs >= 0) {
+                emit_group(hdr, groups, bs, be);
+                bs = -1;
+            }
+            if (open) {
+                bs = cs; be = ce;
+                for (int q = 0; q < 3; q++) { blo[q] = cb[q]; bhi[q] = cb[3 + q]; }
+            }
+        }
+        if (bs >= 0) emit_group(hdr, groups, bs, be);
+    }
+}
+
+// ----------------------------------------------------------- traversal ----
 struct TravArgs {
-    const TreeHeader* hdr;
+    TreeHeader* hdr;
     const int4* info;
     const double4* cm;
-    const uint64_t* keys;
     const int32_t* order;
+    const int2* groups;
     const double* sorted;    // x, y, z, m, r blocks of `cap`
     int cap;
     double* ax;
@@ -48,10 +138,7 @@ struct TravArgs {
     double theta2;
     int levels;
     int shell;
-    int split_depth;
-    int nchunks;
-    int per_xcd;
-    unsigned long long* counters;   // [0] cell terms, [1] body terms
+    unsigned long long* counters;   // [0] cell, [1] body, [2] accumulated terms
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -77,23 +164,6 @@ __device__ __forceinline__ int wscan_incl(int v) {
         if (lane >= o) v += t;
     }
     return v;
-}
-
-struct Box {
-    double lo[3], hi[3];
-};
-
-__device__ __forceinline__ Box wave_box(bool in, double x, double y, double z) {
-    const double big = 1.7976931348623157e308;
-    Box b;
-    b.lo[0] = wmin(in ? x : big); b.lo[1] = wmin(in ? y : big); b.lo[2] = wmin(in ? z : big);
-    b.hi[0] = wmax(in ? x : -big); b.hi[1] = wmax(in ? y : -big); b.hi[2] = wmax(in ? z : -big);
-    return b;
-}
-
-__device__ __forceinline__ double diag2(const Box& b) {
-    double dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
-    return dx * dx + dy * dy + dz * dz;
 }
 
 __device__ __forceinline__ double gk_coeff(double dist_sq, double m, double r,
@@ -136,11 +206,7 @@ __device__ __forceinline__ float gk_coeff_f(float dist_sq, float m, float r,
 }
 
 template <int PREC>
-struct Staging;
-template <>
-struct Staging<CM_PREC_FP64> { double4 v[kWave]; double r[kWave]; int id[kWave]; };
-template <>
-struct Staging<CM_PREC_FP64_FAST> { double4 v[kWave]; double r[kWave]; int id[kWave]; };
+struct Staging { double4 v[kWave]; double r[kWave]; int id[kWave]; };
 template <>
 struct Staging<CM_PREC_FP32_SPLIT> { float4 v[kWave]; float r[kWave]; int id[kWave]; };
 
@@ -152,17 +218,11 @@ k_traverse(TravArgs a) {
     __shared__ int s_part[kWave];
     __shared__ int s_lpref[kWave];
     __shared__ int s_lstart[kWave];
-    __shared__ int s_work[3 * 8];
+    __shared__ int s_g;
     __shared__ Staging<PREC> st;
 
     const int lane = lane_id();
-    const int bid = blockIdx.x;
-    const int chunk = (bid % 8) * a.per_xcd + bid / 8;
-    if (chunk >= a.nchunks) return;
-    const int nA = a.hdr->n_active;
-    const int g0 = chunk * kWave;
-    if (g0 >= nA) return;
-    const int g1 = min(g0 + kWave, nA);
+    const int ngroups = a.hdr->n_groups;
     const double h = a.hdr->box[3];
     const int cap = a.cap;
     const double* sx = a.sorted;
@@ -170,71 +230,41 @@ k_traverse(TravArgs a) {
     const double* sz = a.sorted + 2 * (size_t)cap;
     const double* sm = a.sorted + 3 * (size_t)cap;
     const double* sr = a.sorted + 4 * (size_t)cap;
-
-    const int my = g0 + lane;
-    const bool inchunk = my < g1;
-    double xi = 0.0, yi = 0.0, zi = 0.0;
-    uint64_t key = 0ull;
-    if (inchunk) { xi = sx[my]; yi = sy[my]; zi = sz[my]; key = a.keys[my]; }
     const bool shell = a.shell != 0;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int reserve = 8 * (a.levels + 2);
-
-    // per-lane accumulators (members of the current sub-group only)
-    double acx = 0.0, acy = 0.0, acz = 0.0;   // monopole sums (FP64)
-    double apx = 0.0, apy = 0.0, apz = 0.0;   // body sums
     unsigned long long n_cell = 0ull, n_body = 0ull;
 
-    int nwork = 1;
-    if (lane == 0) { s_work[0] = g0; s_work[1] = g1; s_work[2] = 0; }
-    __syncthreads();
-    while (nwork > 0) {
-        nwork--;
-        const int s = s_work[3 * nwork], e = s_work[3 * nwork + 1], d = s_work[3 * nwork + 2];
+    for (;;) {
+        if (lane == 0) s_g = (int)atomicAdd(&a.hdr->work_next, 1u);
         __syncthreads();
-        bool split = false;
-        if (d < a.split_depth && e - s >= 2) {
-            uint64_t k0 = __shfl(key, s - g0, kWave);
-            uint64_t k1 = __shfl(key, e - 1 - g0, kWave);
-            if (k0 != k1) {
-                int p = 63 - __clzll((long long)(k0 ^ k1));
-                bool inr = my >= s && my < e;
-                uint64_t hb = __ballot(inr && ((key >> p) & 1ull));
-                int mid = g0 + (__ffsll((unsigned long long)hb) - 1);
-                Box all = wave_box(inr, xi, yi, zi);
-                Box b1 = wave_box(inr && my < mid, xi, yi, zi);
-                Box b2 = wave_box(inr && my >= mid, xi, yi, zi);
-                double da = diag2(all), d1 = diag2(b1), d2 = diag2(b2);
-                double dm = d1 > d2 ? d1 : d2;
-                if (da > 4.0 * dm) {
-                    split = true;
-                    if (lane == 0) {
-                        s_work[3 * nwork] = mid; s_work[3 * nwork + 1] = e; s_work[3 * nwork + 2] = d + 1;
-                        s_work[3 * nwork + 3] = s; s_work[3 * nwork + 4] = mid; s_work[3 * nwork + 5] = d + 1;
-                    }
-                    nwork += 2;
-                }
-            }
-        }
+        const int g = s_g;
         __syncthreads();
-        if (split) continue;
-
-        // ---------------- traverse for sub-group [s, e) ----------------
-        const bool member = my >= s && my < e;
+        if (g >= ngroups) break;
+        const int2 gr = a.groups[g];
+        const int s = gr.x, e = gr.y;
         const int nmem = e - s;
-        Box gb = wave_box(member, xi, yi, zi);
-        const double cxg = (gb.lo[0] + gb.hi[0]) * 0.5;
-        const double cyg = (gb.lo[1] + gb.hi[1]) * 0.5;
-        const double czg = (gb.lo[2] + gb.hi[2]) * 0.5;
+        const int my = s + lane;
+        const bool member = lane < nmem;
+        double xi = 0.0, yi = 0.0, zi = 0.0;
+        if (member) { xi = sx[my]; yi = sy[my]; zi = sz[my]; }
+
+        const double big = 1.7976931348623157e308;
+        double lo0 = wmin(member ? xi : big), lo1 = wmin(member ? yi : big),
+               lo2 = wmin(member ? zi : big);
+        double hi0 = wmax(member ? xi : -big), hi1 = wmax(member ? yi : -big),
+               hi2 = wmax(member ? zi : -big);
+        const double cxg = (lo0 + hi0) * 0.5;
+        const double cyg = (lo1 + hi1) * 0.5;
+        const double czg = (lo2 + hi2) * 0.5;
         const float txf = (float)(xi - cxg), tyf = (float)(yi - cyg), tzf = (float)(zi - czg);
-        double fpx = 0.0, fpy = 0.0, fpz = 0.0;   // FP32 path: folded partials
+        double acx = 0.0, acy = 0.0, acz = 0.0;   // monopole sums
+        double apx = 0.0, apy = 0.0, apz = 0.0;   // body sums (FP32: folded partials)
 
         auto flush = [&](int cnt, bool cells) {
-            // stage one entry per lane
             if (lane < cnt) {
                 if (cells) {
-                    int k = s_cell[lane];
-                    double4 v = a.cm[k];
+                    double4 v = a.cm[s_cell[lane]];
                     if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                         st.v[lane] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
                                                  (float)(v.z - czg), (float)v.w);
@@ -266,7 +296,7 @@ k_traverse(TravArgs a) {
                     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
                     const float e2 = (float)a.eps2;
                     if (cells || (!shell && e2 > 0.0f)) {
-                        // self term (dx = 0) is exactly zero when eps2 > 0
+                        // with eps2 > 0 the self term (dx = 0) is exactly zero
                         for (int j = 0; j < cnt; j++) {
                             float4 v = st.v[j];
                             float dx = txf - v.x, dy = tyf - v.y, dz = tzf - v.z;
@@ -289,7 +319,7 @@ k_traverse(TravArgs a) {
                             gz = __builtin_fmaf(-f, dz, gz);
                         }
                     }
-                    fpx += (double)gx; fpy += (double)gy; fpz += (double)gz;
+                    apx += (double)gx; apy += (double)gy; apz += (double)gz;
                 } else if (cells) {
                     for (int j = 0; j < cnt; j++) {
                         double4 v = st.v[j];
@@ -343,13 +373,13 @@ k_traverse(TravArgs a) {
             int4 nd = make_int4(-1, 0, 0, 0);
             if (node >= 0) {
                 double4 c = a.cm[node];
+                nd = a.info[node];
                 if (c.w != 0.0) {
-                    nd = a.info[node];
                     const int level = nd.y >> 8;
                     const bool leaf = nd.x < 0;
-                    double ddx = c.x < gb.lo[0] ? gb.lo[0] - c.x : (c.x > gb.hi[0] ? c.x - gb.hi[0] : 0.0);
-                    double ddy = c.y < gb.lo[1] ? gb.lo[1] - c.y : (c.y > gb.hi[1] ? c.y - gb.hi[1] : 0.0);
-                    double ddz = c.z < gb.lo[2] ? gb.lo[2] - c.z : (c.z > gb.hi[2] ? c.z - gb.hi[2] : 0.0);
+                    double ddx = c.x < lo0 ? lo0 - c.x : (c.x > hi0 ? c.x - hi0 : 0.0);
+                    double ddy = c.y < lo1 ? lo1 - c.y : (c.y > hi1 ? c.y - hi1 : 0.0);
+                    double ddz = c.z < lo2 ? lo2 - c.z : (c.z > hi2 ? c.z - hi2 : 0.0);
                     double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
                     double hl = ldexp(h, -level);
                     double size2 = 4.0 * hl * hl;
@@ -357,7 +387,7 @@ k_traverse(TravArgs a) {
                     else kind = leaf ? 2 : 3;
                 }
             }
-            // accepted cells
+            // accepted cells, in lane order
             uint64_t cm_ = __ballot(kind == 1);
             if (kind == 1) s_cell[ncell + __popcll(cm_ & lt)] = node;
             ncell += __popcll(cm_);
@@ -409,13 +439,11 @@ k_traverse(TravArgs a) {
         if (member) {
             const int o = a.order[my];
             if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                a.ax[o] = fpx; a.ay[o] = fpy; a.az[o] = fpz;
+                a.ax[o] = apx; a.ay[o] = apy; a.az[o] = apz;
             } else {
                 a.ax[o] = acx + apx; a.ay[o] = acy + apy; a.az[o] = acz + apz;
             }
         }
-        acx = acy = acz = apx = apy = apz = 0.0;
-        __syncthreads();
     }
     if (lane == 0) {
         atomicAdd(&a.counters[0], n_cell);
@@ -446,14 +474,17 @@ int launch_barnes_hut(Engine& e, double softening) {
     const int32_t* order = nullptr;
     int rc = build_tree(e, &keys, &order);
     if (rc != CM_OK) return rc;
+    int gb = std::min(1024, std::max(1, (n + 255) / 256));
+    hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, st, e.hdr, e.nd_info,
+                       e.nd_box, e.cfg.group_size, e.groups);
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[3], st));
     e.mark(1);
     TravArgs a;
     a.hdr = e.hdr;
     a.info = e.nd_info;
     a.cm = e.nd_cm;
-    a.keys = keys;
     a.order = order;
+    a.groups = e.groups;
     a.sorted = e.sorted;
     a.cap = e.capacity;
     a.ax = e.f[AX]; a.ay = e.f[AY]; a.az = e.f[AZ];
@@ -462,11 +493,9 @@ int launch_barnes_hut(Engine& e, double softening) {
     a.theta2 = th * th;
     a.levels = e.cfg.morton_levels;
     a.shell = e.cfg.shell_theorem;
-    a.split_depth = e.cfg.group_split_depth;
-    a.nchunks = (n + kWave - 1) / kWave;
-    a.per_xcd = (a.nchunks + 7) / 8;
     a.counters = &e.hdr->n_cell_int;
-    int nblocks = a.per_xcd * 8;
+    // persistent wavefronts: about 16 per CU, never more than bodies / 8
+    int nblocks = std::max(1, std::min(4096, (n + 7) / 8));
     switch (e.cfg.precision) {
         case CM_PREC_FP64:
             hipLaunchKernelGGL(k_traverse<CM_PREC_FP64>, dim3(nblocks), dim3(kWave), 0, st, a);

@@ -121,6 +121,8 @@ k_bbox_final(int nblocks, const double* __restrict__ part,
         hdr->level_start[1] = 1;
         hdr->n_cell_int = 0ull;
         hdr->n_body_int = 0ull;
+        hdr->n_groups = 0;
+        hdr->work_next = 0u;
         nd_info[0] = make_int4(-1, 0, 0, na);
     }
 }
@@ -182,34 +184,48 @@ k_sort_hist(int n, int shift, int ntiles, const uint64_t* __restrict__ keys,
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (256 x ntiles) in one block.
-__global__ void __launch_bounds__(1024)
-k_sort_scan(int total, uint32_t* __restrict__ hist) {
-    __shared__ uint32_t part[1024];
+// Block-wide exclusive scan of one value per thread (kSortThreads threads).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
+                                                    uint32_t* total) {
     int t = threadIdx.x;
-    int per = (total + 1023) / 1024;
-    int b = t * per, e = min(total, b + per);
-    uint32_t s = 0;
-    for (int i = b; i < e; i++) s += hist[i];
-    part[t] = s;
+    s_tmp[t] = v;
     __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = t >= o ? part[t - o] : 0u;
+    for (int o = 1; o < kSortThreads; o <<= 1) {
+        uint32_t x = t >= o ? s_tmp[t - o] : 0u;
         __syncthreads();
-        part[t] += v;
+        s_tmp[t] += x;
         __syncthreads();
     }
-    uint32_t run = t ? part[t - 1] : 0u;
+    uint32_t incl = s_tmp[t];
+    *total = s_tmp[kSortThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// One block per digit: exclusive scan of that digit's per-tile counts along
+// the tiles; the digit's total goes to totals[digit].
+__global__ void __launch_bounds__(kSortThreads)
+k_sort_scan(int ntiles, uint32_t* __restrict__ hist, uint32_t* __restrict__ totals) {
+    __shared__ uint32_t s_tmp[kSortThreads];
+    uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+    int per = (ntiles + kSortThreads - 1) / kSortThreads;
+    int b = threadIdx.x * per, e = min(ntiles, b + per);
+    uint32_t s = 0;
+    for (int i = b; i < e; i++) s += row[i];
+    uint32_t tot;
+    uint32_t run = block_excl_scan(s, s_tmp, &tot);
     for (int i = b; i < e; i++) {
-        uint32_t v = hist[i];
-        hist[i] = run;
+        uint32_t v = row[i];
+        row[i] = run;
         run += v;
     }
+    if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
                const int32_t* __restrict__ vin, const uint32_t* __restrict__ hist,
+               const uint32_t* __restrict__ totals,
                uint64_t* __restrict__ kout, int32_t* __restrict__ vout) {
     constexpr int kWaves = kSortThreads / kWave;
     __shared__ uint32_t base[256];
@@ -218,7 +234,11 @@ k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
     int t = threadIdx.x;
     int lane = t & (kWave - 1);
     int w = t / kWave;
-    base[t] = hist[t * ntiles + blockIdx.x];
+    {   // digit base = all smaller digits' totals + this digit in earlier tiles
+        uint32_t tot;
+        uint32_t dbase = block_excl_scan(totals[t], run, &tot);
+        base[t] = dbase + hist[(size_t)t * ntiles + blockIdx.x];
+    }
     run[t] = 0u;
     uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int tile0 = blockIdx.x * kSortTile;
@@ -329,9 +349,24 @@ k_build_level(int L, int levels, int leaf_cap, int node_cap, int btiles,
         if (valid && me.w > leaf_cap && L < levels) {
             int shift = 3 * (levels - L - 1);
             int lo = me.z, hi = me.z + me.w;
-            int b[9];
+            // the 7 boundaries (first index with digit >= d) are searched
+            // together, one step of each per round, so their loads overlap
+            int b[9], blo[8], bhi[8];
             b[0] = lo; b[8] = hi;
-            for (uint32_t d = 1; d < 8; d++) b[d] = lower_digit(keys, b[d - 1], hi, shift, d);
+            for (int d = 1; d < 8; d++) { blo[d] = lo; bhi[d] = hi; }
+            for (int round = 0; round < 32; round++) {
+                bool any = false;
+                for (int d = 1; d < 8; d++) {
+                    if (blo[d] < bhi[d]) {
+                        int mid = (blo[d] + bhi[d]) >> 1;
+                        if (((uint32_t)(keys[mid] >> shift) & 7u) < (uint32_t)d) blo[d] = mid + 1;
+                        else bhi[d] = mid;
+                        any = true;
+                    }
+                }
+                if (!any) break;
+            }
+            for (int d = 1; d < 8; d++) b[d] = blo[d];
             for (int d = 0; d < 8; d++) {
                 int c = b[d + 1] - b[d];
                 if (c > 0) { cs[nch] = b[d]; cc[nch] = c; nch++; }
@@ -347,30 +382,41 @@ k_build_level(int L, int levels, int leaf_cap, int node_cap, int btiles,
             agg += s_wsum[q];
         }
         int excl = woff + incl - nch;
-        // decoupled look-back over tiles of this level
-        if (t == 0) {
+        // decoupled look-back over tiles of this level, 64 predecessors per
+        // round (one per lane of the first wavefront)
+        if (w == 0) {
             unsigned long long prefix = 0ull;
             if (tile == 0) {
-                __hip_atomic_store(&status[0], (2ull << 32) | (unsigned)agg,
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0)
+                    __hip_atomic_store(&status[0], (2ull << 32) | (unsigned)agg,
+                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                __hip_atomic_store(&status[tile], (1ull << 32) | (unsigned)agg,
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0)
+                    __hip_atomic_store(&status[tile], (1ull << 32) | (unsigned)agg,
+                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
                 int p = tile - 1;
                 for (;;) {
-                    unsigned long long v = __hip_atomic_load(
-                        &status[p], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    int idx = p - lane;
+                    unsigned long long v = idx >= 0
+                        ? __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE,
+                                            __HIP_MEMORY_SCOPE_AGENT)
+                        : (2ull << 32);
                     unsigned flag = (unsigned)(v >> 32);
-                    if (flag == 0u) continue;
-                    prefix += v & 0xffffffffull;
-                    if (flag == 2u) break;
-                    p--;
+                    if (__ballot(flag == 0u)) continue;     // a predecessor not yet published
+                    uint64_t incl = __ballot(flag == 2u);
+                    int stop = incl ? __ffsll((unsigned long long)incl) - 1 : kWave - 1;
+                    unsigned long long mine = lane <= stop ? (v & 0xffffffffull) : 0ull;
+                    for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, kWave);
+                    prefix += mine;
+                    if (incl) break;
+                    p -= kWave;
                 }
-                __hip_atomic_store(&status[tile],
-                                   (2ull << 32) | (unsigned)(prefix + (unsigned)agg),
-                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0)
+                    __hip_atomic_store(&status[tile],
+                                       (2ull << 32) | (unsigned)(prefix + (unsigned)agg),
+                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             }
-            s_prefix = (int)prefix;
+            if (lane == 0) s_prefix = (int)prefix;
         }
         __syncthreads();
         int first = le + s_prefix + excl;
@@ -395,27 +441,34 @@ k_build_level(int L, int levels, int leaf_cap, int node_cap, int btiles,
     }
 }
 
-// Bottom-up mass and centre of mass for the nodes of level L.
+// Bottom-up mass, centre of mass and body bounding box for the nodes of
+// level L (box: 6 doubles lo x, y, z, hi x, y, z per node).
 __global__ void __launch_bounds__(256)
 k_mass_level(int L, const TreeHeader* __restrict__ hdr,
              const int4* __restrict__ info, const double* __restrict__ sorted,
-             int cap, double4* __restrict__ cm, double* __restrict__ sums) {
+             int cap, double4* __restrict__ cm, double* __restrict__ sums,
+             double* __restrict__ box) {
     const int ls = hdr->level_start[L];
     const int le = hdr->level_start[L + 1];
     const double* sx = sorted;
     const double* sy = sorted + cap;
     const double* sz = sorted + 2 * (size_t)cap;
     const double* sm = sorted + 3 * (size_t)cap;
+    const double big = 1.7976931348623157e308;
     for (int k = ls + blockIdx.x * 256 + threadIdx.x; k < le; k += gridDim.x * 256) {
         int4 nd = info[k];
         double M = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+        double lo0 = big, lo1 = big, lo2 = big, hi0 = -big, hi1 = -big, hi2 = -big;
         if (nd.x < 0) {
             for (int q = nd.z; q < nd.z + nd.w; q++) {
                 double mb = sm[q];
+                double x = sx[q], y = sy[q], z = sz[q];
                 M += mb;
-                ax += mb * sx[q];
-                ay += mb * sy[q];
-                az += mb * sz[q];
+                ax += mb * x;
+                ay += mb * y;
+                az += mb * z;
+                lo0 = x < lo0 ? x : lo0; lo1 = y < lo1 ? y : lo1; lo2 = z < lo2 ? z : lo2;
+                hi0 = x > hi0 ? x : hi0; hi1 = y > hi1 ? y : hi1; hi2 = z > hi2 ? z : hi2;
             }
         } else {
             int nch = nd.y & 255;
@@ -424,8 +477,14 @@ k_mass_level(int L, const TreeHeader* __restrict__ hdr,
                 ax += sums[3 * (size_t)c];
                 ay += sums[3 * (size_t)c + 1];
                 az += sums[3 * (size_t)c + 2];
+                const double* cb = box + 6 * (size_t)c;
+                lo0 = cb[0] < lo0 ? cb[0] : lo0; lo1 = cb[1] < lo1 ? cb[1] : lo1;
+                lo2 = cb[2] < lo2 ? cb[2] : lo2; hi0 = cb[3] > hi0 ? cb[3] : hi0;
+                hi1 = cb[4] > hi1 ? cb[4] : hi1; hi2 = cb[5] > hi2 ? cb[5] : hi2;
             }
         }
+        double* kb = box + 6 * (size_t)k;
+        kb[0] = lo0; kb[1] = lo1; kb[2] = lo2; kb[3] = hi0; kb[4] = hi1; kb[5] = hi2;
         sums[3 * (size_t)k] = ax;
         sums[3 * (size_t)k + 1] = ay;
         sums[3 * (size_t)k + 2] = az;
@@ -440,6 +499,29 @@ k_mass_level(int L, const TreeHeader* __restrict__ hdr,
 __global__ void k_fill_u64(size_t n, unsigned long long* __restrict__ p) {
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     for (; i < n; i += (size_t)gridDim.x * 256) p[i] = 0ull;
+}
+
+// Stable LSD radix sort of n (key, value) pairs over 8-bit digits
+// first_pass .. first_pass + passes - 1.  On return *k/*v hold the sorted
+// pairs (the buffers are swapped after every pass); hist holds
+// 256 * (ceil(n / kSortTile) + 1) words.
+void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
+                      uint64_t** k, int32_t** v, uint64_t** kt, int32_t** vt,
+                      uint32_t* hist) {
+    if (n <= 0) return;
+    int ntiles = (n + kSortTile - 1) / kSortTile;
+    uint32_t* totals = hist + (size_t)256 * ntiles;
+    for (int p = first_pass; p < first_pass + passes; p++) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, n,
+                           shift, ntiles, *k, hist);
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kSortThreads), 0, st, ntiles,
+                           hist, totals);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortThreads), 0, st,
+                           n, shift, ntiles, *k, *v, hist, totals, *kt, *vt);
+        std::swap(*k, *kt);
+        std::swap(*v, *vt);
+    }
 }
 
 // Builds the tree for the current state; leaves keys/order in keys_a/vals_a
@@ -458,20 +540,9 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
                        e.f[PX], e.f[PY], e.f[PZ], e.act, e.keys_a, e.vals_a);
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
     int passes = (3 * levels + 1 + 7) / 8;
-    int ntiles = (n + kSortTile - 1) / kSortTile;
     uint64_t* ka = e.keys_a; uint64_t* kb = e.keys_b;
     int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
-    for (int p = 0; p < passes; p++) {
-        int shift = 8 * p;
-        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, n,
-                           shift, ntiles, ka, e.hist);
-        hipLaunchKernelGGL(k_sort_scan, dim3(1), dim3(1024), 0, st, 256 * ntiles,
-                           e.hist);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortThreads), 0, st,
-                           n, shift, ntiles, ka, va, e.hist, kb, vb);
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
+    radix_sort_pairs(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist);
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
     hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.f[PX],
                        e.f[PY], e.f[PZ], e.f[MASS], e.f[RAD], e.sorted, e.capacity);
@@ -489,7 +560,7 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
         double maxsz = std::min((double)n, L < 20 ? std::ldexp(1.0, 3 * L) : 1e18);
         int g = (int)std::min(1024.0, std::max(1.0, std::ceil(maxsz / 256)));
         hipLaunchKernelGGL(k_mass_level, dim3(g), dim3(256), 0, st, L, e.hdr,
-                           e.nd_info, e.sorted, e.capacity, e.nd_cm, e.nd_sum);
+                           e.nd_info, e.sorted, e.capacity, e.nd_cm, e.nd_sum, e.nd_box);
     }
     CM_HIP_TRY(e, hipGetLastError());
     *keys_out = ka;

@@ -88,3 +88,105 @@ extern "C" int32_t cm_generate_plummer(int32_t n, double total_mass,
     }
     return CM_OK;
 }
+
+namespace {
+
+// Kepler elements -> Cartesian state about a central mass mu (G = 1).
+void kepler_to_cartesian(double mu, double a, double e, double inc, double node,
+                         double peri, double mean_anom, double* r, double* v) {
+    double E = mean_anom;
+    for (int it = 0; it < 30; it++) {
+        double f = E - e * std::sin(E) - mean_anom;
+        double d = 1.0 - e * std::cos(E);
+        double step = f / d;
+        E -= step;
+        if (std::fabs(step) < 1e-15) break;
+    }
+    double cE = std::cos(E), sE = std::sin(E);
+    double b = std::sqrt(1.0 - e * e);
+    double xo = a * (cE - e), yo = a * b * sE;
+    double nmean = std::sqrt(mu / (a * a * a));
+    double denom = 1.0 - e * cE;
+    double vxo = -a * nmean * sE / denom, vyo = a * nmean * b * cE / denom;
+    double cO = std::cos(node), sO = std::sin(node);
+    double cw = std::cos(peri), sw = std::sin(peri);
+    double ci = std::cos(inc), si = std::sin(inc);
+    double r11 = cO * cw - sO * sw * ci, r12 = -cO * sw - sO * cw * ci;
+    double r21 = sO * cw + cO * sw * ci, r22 = -sO * sw + cO * cw * ci;
+    double r31 = sw * si, r32 = cw * si;
+    r[0] = r11 * xo + r12 * yo; r[1] = r21 * xo + r22 * yo; r[2] = r31 * xo + r32 * yo;
+    v[0] = r11 * vxo + r12 * vyo; v[1] = r21 * vxo + r22 * vyo; v[2] = r31 * vxo + r32 * vyo;
+}
+
+double rayleigh(SplitMix64& rng, double sigma) {
+    return sigma * std::sqrt(-2.0 * std::log(1.0 - rng.uniform()));
+}
+
+void com_shift(int32_t n, const cm_soa_ptrs* out) {
+    long double m = 0, c[3] = {0, 0, 0}, u[3] = {0, 0, 0};
+    for (int32_t i = 0; i < n; i++) {
+        long double mi = out->mass[i];
+        m += mi;
+        c[0] += mi * out->pos_x[i]; c[1] += mi * out->pos_y[i]; c[2] += mi * out->pos_z[i];
+        u[0] += mi * out->vel_x[i]; u[1] += mi * out->vel_y[i]; u[2] += mi * out->vel_z[i];
+    }
+    double p[3], w[3];
+    for (int k = 0; k < 3; k++) { p[k] = (double)(c[k] / m); w[k] = (double)(u[k] / m); }
+    for (int32_t i = 0; i < n; i++) {
+        out->pos_x[i] -= p[0]; out->pos_y[i] -= p[1]; out->pos_z[i] -= p[2];
+        out->vel_x[i] -= w[0]; out->vel_y[i] -= w[1]; out->vel_z[i] -= w[2];
+    }
+}
+
+}  // namespace
+
+// Cold thin disk around a central star (the collision workload): body 0 is a
+// 1 Msun star of the given radius (density consistent with that radius);
+// bodies 1..n-1 have semi-major axes log-uniform in [a_min, a_max],
+// Rayleigh eccentricity (sigma_e) and inclination (sigma_i), uniform angles,
+// masses log-uniform in [m_min, m_max] and radii from the given density
+// (Msun / AU^3) by DensityModel.ComputeRadius (Types/DensityModel.cs:60-66).
+extern "C" int32_t cm_generate_disk(int32_t n, uint64_t seed, double star_radius,
+                                    double a_min, double a_max, double sigma_e,
+                                    double sigma_i, double m_min, double m_max,
+                                    double density, const cm_soa_ptrs* out) {
+    if (n < 1 || !out || !out->pos_x || !out->vel_x || !out->mass || !out->radius ||
+        !out->density || !out->is_active || !out->is_collidable || !out->body_type)
+        return CM_ERR_INVALID_ARGUMENT;
+    if (!(a_min > 0.0) || !(a_max >= a_min) || !(m_min > 0.0) || !(m_max >= m_min) ||
+        !(density > 0.0) || !(star_radius > 0.0))
+        return CM_ERR_INVALID_ARGUMENT;
+    SplitMix64 rng(seed);
+    const double mstar = 1.0;
+    out->pos_x[0] = out->pos_y[0] = out->pos_z[0] = 0.0;
+    out->vel_x[0] = out->vel_y[0] = out->vel_z[0] = 0.0;
+    out->mass[0] = mstar;
+    out->radius[0] = star_radius;
+    out->density[0] = 3.0 * mstar / (4.0 * kPi * star_radius * star_radius * star_radius);
+    out->body_type[0] = 0;   // Star
+    out->is_active[0] = 1;
+    out->is_collidable[0] = 1;
+    const double la = std::log(a_min), lb = std::log(a_max);
+    const double lm0 = std::log(m_min), lm1 = std::log(m_max);
+    for (int32_t i = 1; i < n; i++) {
+        double a = std::exp(la + (lb - la) * rng.uniform());
+        double e = std::min(rayleigh(rng, sigma_e), 0.9);
+        double inc = rayleigh(rng, sigma_i);
+        double node = 2.0 * kPi * rng.uniform();
+        double peri = 2.0 * kPi * rng.uniform();
+        double mean = 2.0 * kPi * rng.uniform();
+        double m = std::exp(lm0 + (lm1 - lm0) * rng.uniform());
+        double r[3], v[3];
+        kepler_to_cartesian(mstar + m, a, e, inc, node, peri, mean, r, v);
+        out->pos_x[i] = r[0]; out->pos_y[i] = r[1]; out->pos_z[i] = r[2];
+        out->vel_x[i] = v[0]; out->vel_y[i] = v[1]; out->vel_z[i] = v[2];
+        out->mass[i] = m;
+        out->density[i] = density;
+        out->radius[i] = std::cbrt(3.0 * (m / density) / (4.0 * kPi));
+        out->body_type[i] = 5;   // Asteroid
+        out->is_active[i] = 1;
+        out->is_collidable[i] = 1;
+    }
+    com_shift(n, out);
+    return CM_OK;
+}

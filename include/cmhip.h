@@ -68,7 +68,7 @@ typedef struct {
     int32_t morton_levels;     /* key bits / 3: 10 (30-bit keys) .. 21 */
     double theta;              /* BarnesHutBackend.Theta, clamped to >= 0.2 */
     int32_t leaf_capacity;     /* bodies per leaf bucket, 1..64 */
-    int32_t group_split_depth; /* target-group split depth, 0..3 */
+    int32_t group_size;        /* max bodies per target group (one wavefront), 1..64 */
 } cm_backend_config;
 
 /* Pointers to the 18 BodySoA arrays (SoA/BodySoA.cs:42-69).  On upload a
@@ -192,6 +192,19 @@ int32_t cm_compact(void* engine, int32_t* new_count, int32_t* old_index);
  * SplitMix64 stream; mass_fraction_cut < 1 truncates the sphere at the radius
  * enclosing that mass fraction; com_frame subtracts the centre-of-mass
  * position and velocity.  Writes pos/vel/mass/radius of n bodies. */
+/* Cold thin disk around a 1 Msun star (no reference generator exists for it;
+ * it stands in for the reference's body setup of collision scenarios):
+ * body 0 is the star (radius star_radius, density consistent with it),
+ * bodies 1..n-1 have log-uniform semi-major axes in [a_min, a_max], Rayleigh
+ * eccentricity / inclination, uniform angles, log-uniform masses in
+ * [m_min, m_max] and radii cbrt(3 (m / density) / (4 pi))
+ * (DensityModel.ComputeRadius, Types/DensityModel.cs:60-66); all active and
+ * collidable, centre-of-mass frame.  Writes pos, vel, mass, radius, density,
+ * is_active, is_collidable and body_type. */
+int32_t cm_generate_disk(int32_t n, uint64_t seed, double star_radius,
+                         double a_min, double a_max, double sigma_e,
+                         double sigma_i, double m_min, double m_max,
+                         double density, const cm_soa_ptrs* out);
 int32_t cm_generate_plummer(int32_t n, double total_mass, double scale_length,
                             uint64_t seed, double mass_fraction_cut,
                             int32_t com_frame, const cm_soa_ptrs* out);

@@ -608,6 +608,13 @@ int cmo_tree_build(int n, const double* px, const double* py, const double* pz,
  * arithmetic) and, separately, the listed bodies other than itself
  * (GravityKernel arithmetic, BarnesHutBackend.cs:415-425); a = monopoles + bodies.
  * theta is clamped to >= 0.2 (BarnesHutBackend.cs:129). */
+/* walk statistics of the last cmo_bh_tree calls: sub-groups, walk steps,
+ * cell-list flush slots, body-list flush slots (64-entry flushes) */
+static int64_t g_walk[4];
+void cmo_walk_stats(int64_t* out, int reset) {
+    for (int k = 0; k < 4; k++) { out[k] = g_walk[k]; if (reset) g_walk[k] = 0; }
+}
+
 static void group_box(const int32_t* ord, const double* px, const double* py,
                       const double* pz, int s, int e, double* lo, double* hi) {
     lo[0] = lo[1] = lo[2] = DBL_MAX;
@@ -628,10 +635,16 @@ static double box_diag2(const double* lo, const double* hi) {
     return dx * dx + dy * dy + dz * dz;
 }
 
+static double box_vol(const double* lo, const double* hi, double pad) {
+    return ((hi[0] - lo[0]) + pad) * ((hi[1] - lo[1]) + pad) * ((hi[2] - lo[2]) + pad);
+}
+
 /* Split the sorted range [s, e) of one target group at its highest differing
  * key bit while the two halves are far apart: the split is taken when the
- * squared diagonal of the whole box exceeds 4x the larger half's, up to
- * max_depth times.  Emitted sub-ranges are appended to out (pairs). */
+ * squared diagonal of the whole box exceeds 4x the larger half's, or its
+ * volume exceeds 4x the halves' summed volumes (extents padded by 1e-3 of the
+ * whole box's summed extents), up to max_depth times.  Emitted sub-ranges are
+ * appended to out (pairs). */
 static int split_group(const uint64_t* ks, const int32_t* ord, const double* px,
                        const double* py, const double* pz, int s, int e,
                        int depth, int max_depth, int32_t* out, int nout) {
@@ -646,7 +659,9 @@ static int split_group(const uint64_t* ks, const int32_t* ord, const double* px,
         group_box(ord, px, py, pz, mid, e, lo2, hi2);
         double d = box_diag2(lo, hi), d1 = box_diag2(lo1, hi1), d2 = box_diag2(lo2, hi2);
         double dm = d1 > d2 ? d1 : d2;
-        if (d > 4.0 * dm) {
+        double pad = 1e-3 * ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
+        double v = box_vol(lo, hi, pad), v1 = box_vol(lo1, hi1, pad), v2 = box_vol(lo2, hi2, pad);
+        if (d > 4.0 * dm || v > 4.0 * (v1 + v2)) {
             nout = split_group(ks, ord, px, py, pz, s, mid, depth + 1, max_depth, out, nout);
             return split_group(ks, ord, px, py, pz, mid, e, depth + 1, max_depth, out, nout);
         }
@@ -654,6 +669,162 @@ static int split_group(const uint64_t* ks, const int32_t* ord, const double* px,
     out[2 * nout] = s;
     out[2 * nout + 1] = e;
     return nout + 1;
+}
+
+static int cmp_range(const void* a, const void* b) {
+    int x = ((const int32_t*)a)[0], y = ((const int32_t*)b)[0];
+    return (x > y) - (x < y);
+}
+
+/* Partition of the na sorted active bodies into target groups of at most
+ * group_size.  policy >= 0: consecutive chunks of group_size, each split by
+ * split_group with max depth = policy.  policy == -1: the maximal tree nodes
+ * holding at most group_size bodies.  policy == -2: those nodes, with
+ * consecutive ones merged (left to right, greedily) while the merged count
+ * fits and split_group's rule would not separate them again.
+ * Returns a malloc'd array of (start, end) pairs in ascending order. */
+/* 1 when the sorted range [s, e) should stay split at mid: the whole box's
+ * squared diagonal exceeds 4x the larger part's, or its padded volume
+ * exceeds 4x the parts' summed padded volumes. */
+static int box_split(const int32_t* ord, const double* px, const double* py,
+                     const double* pz, int s, int mid, int e) {
+    double lo[3], hi[3], lo1[3], hi1[3], lo2[3], hi2[3];
+    group_box(ord, px, py, pz, s, e, lo, hi);
+    group_box(ord, px, py, pz, s, mid, lo1, hi1);
+    group_box(ord, px, py, pz, mid, e, lo2, hi2);
+    double d = box_diag2(lo, hi), d1 = box_diag2(lo1, hi1), d2 = box_diag2(lo2, hi2);
+    double dm = d1 > d2 ? d1 : d2;
+    double pad = 1e-3 * ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
+    double v = box_vol(lo, hi, pad), v1 = box_vol(lo1, hi1, pad), v2 = box_vol(lo2, hi2, pad);
+    return d > 4.0 * dm || v > 4.0 * (v1 + v2);
+}
+
+static int common_level(uint64_t a, uint64_t b, int levels) {
+    if (a == b) return levels;
+    int bits = 3 * levels - (64 - __builtin_clzll(a ^ b));   /* shared top bits */
+    return bits / 3;
+}
+
+static int32_t* make_groups(int na, const uint64_t* ks, const int32_t* ord,
+                            const double* px, const double* py, const double* pz,
+                            const tnode* nd, int nn, int group_size, int policy,
+                            int levels, int* nr_out) {
+    int cap = na / 2 + 64;
+    int32_t* r = (int32_t*)malloc((size_t)cap * 2 * 4);
+    int nr = 0;
+    if (policy >= 0) {
+        int32_t sub[2 * 64];
+        for (int g0 = 0; g0 < na; g0 += group_size) {
+            int g1 = g0 + group_size < na ? g0 + group_size : na;
+            int ns = split_group(ks, ord, px, py, pz, g0, g1, 0, policy, sub, 0);
+            for (int k = 0; k < ns; k++) {
+                if (nr == cap) { cap *= 2; r = (int32_t*)realloc(r, (size_t)cap * 8); }
+                r[2 * nr] = sub[2 * k]; r[2 * nr + 1] = sub[2 * k + 1]; nr++;
+            }
+        }
+    } else {
+        for (int k = 0; k < nn; k++) {
+            const tnode* t = &nd[k];
+            if (t->count == 0) continue;
+            int take = 0;
+            if (k == 0) take = t->count <= group_size;
+            if (t->first >= 0 && t->count > group_size) {
+                for (int c = t->first; c < t->first + t->nchild; c++) {
+                    if (nd[c].count <= group_size) {
+                        if (nr == cap) { cap *= 2; r = (int32_t*)realloc(r, (size_t)cap * 8); }
+                        r[2 * nr] = nd[c].start; r[2 * nr + 1] = nd[c].start + nd[c].count; nr++;
+                    }
+                }
+            } else if (take) {
+                r[2 * nr] = t->start; r[2 * nr + 1] = t->start + t->count; nr++;
+            } else if (t->first < 0 && t->count > group_size) {
+                for (int q = t->start; q < t->start + t->count; q += group_size) {
+                    if (nr == cap) { cap *= 2; r = (int32_t*)realloc(r, (size_t)cap * 8); }
+                    int qe = q + group_size < t->start + t->count ? q + group_size
+                                                                  : t->start + t->count;
+                    r[2 * nr] = q; r[2 * nr + 1] = qe; nr++;
+                }
+            }
+        }
+        if (policy == -3) {
+            /* sibling packing: for every node with more than group_size
+             * bodies, runs of consecutive children that hold at most
+             * group_size bodies are packed in child order into bins while the
+             * bin stays within group_size and the box rule keeps it whole */
+            nr = 0;
+            if (nd[0].count <= group_size && nd[0].count > 0) {
+                r[0] = 0; r[1] = nd[0].count; nr = 1;
+            }
+            for (int k = 0; k < nn; k++) {
+                const tnode* t = &nd[k];
+                if (t->count <= group_size) continue;
+                if (t->first < 0) {
+                    /* deepest-level leaf with more than group_size bodies:
+                     * consecutive slices of group_size */
+                    for (int q = t->start; q < t->start + t->count; q += group_size) {
+                        if (nr == cap) { cap *= 2; r = (int32_t*)realloc(r, (size_t)cap * 8); }
+                        int qe = q + group_size < t->start + t->count ? q + group_size
+                                                                      : t->start + t->count;
+                        r[2 * nr] = q; r[2 * nr + 1] = qe; nr++;
+                    }
+                    continue;
+                }
+                int bs = -1, be = -1;
+                for (int c = t->first; c < t->first + t->nchild; c++) {
+                    int cs = nd[c].start, ce = nd[c].start + nd[c].count;
+                    int close = 0, open = 0;
+                    if (nd[c].count > group_size) {
+                        close = 1;
+                    } else if (bs < 0) {
+                        open = 1;
+                    } else if (ce - bs <= group_size && !box_split(ord, px, py, pz, bs, cs, ce)) {
+                        be = ce;
+                    } else {
+                        close = 1; open = 1;
+                    }
+                    if (close && bs >= 0) {
+                        if (nr == cap) { cap *= 2; r = (int32_t*)realloc(r, (size_t)cap * 8); }
+                        r[2 * nr] = bs; r[2 * nr + 1] = be; nr++;
+                        bs = -1;
+                    }
+                    if (open) { bs = cs; be = ce; }
+                }
+                if (bs >= 0) {
+                    if (nr == cap) { cap *= 2; r = (int32_t*)realloc(r, (size_t)cap * 8); }
+                    r[2 * nr] = bs; r[2 * nr + 1] = be; nr++;
+                }
+            }
+        }
+        qsort(r, (size_t)nr, 8, cmp_range);
+        if (policy == -2 && nr > 0) {
+            int w = 0;
+            for (int k = 1; k < nr; k++) {
+                int s = r[2 * w], mid = r[2 * k], e = r[2 * k + 1];
+                int merge = 0;
+                /* tightest key cell of a range: common prefix of its end keys */
+                int lvl_all = common_level(ks[s], ks[e - 1], levels);
+                int lvl_a = common_level(ks[s], ks[mid - 1], levels);
+                int lvl_b = common_level(ks[mid], ks[e - 1], levels);
+                int lvl_min = lvl_a > lvl_b ? lvl_a : lvl_b;   /* the finer one */
+                if (e - s <= group_size && lvl_all >= lvl_min - 1) {
+                    double lo[3], hi[3], lo1[3], hi1[3], lo2[3], hi2[3];
+                    group_box(ord, px, py, pz, s, e, lo, hi);
+                    group_box(ord, px, py, pz, s, mid, lo1, hi1);
+                    group_box(ord, px, py, pz, mid, e, lo2, hi2);
+                    double d = box_diag2(lo, hi), d1 = box_diag2(lo1, hi1), d2 = box_diag2(lo2, hi2);
+                    double dm = d1 > d2 ? d1 : d2;
+                    double pad = 1e-3 * ((hi[0] - lo[0]) + (hi[1] - lo[1]) + (hi[2] - lo[2]));
+                    double v = box_vol(lo, hi, pad), v1 = box_vol(lo1, hi1, pad), v2 = box_vol(lo2, hi2, pad);
+                    merge = !(d > 4.0 * dm || v > 4.0 * (v1 + v2));
+                }
+                if (merge) r[2 * w + 1] = e;
+                else { w++; r[2 * w] = mid; r[2 * w + 1] = e; }
+            }
+            nr = w + 1;
+        }
+    }
+    *nr_out = nr;
+    return r;
 }
 
 int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
@@ -678,7 +849,21 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
     double theta2 = th * th;
     double eps2 = eps * eps;
     double h = box[3];
-    int ng = (na + group_size - 1) / group_size;
+    int nr = 0;
+    int32_t* rng = make_groups(na, ks, ord, px, py, pz, nd, nn, group_size,
+                               split_depth, levels, &nr);
+    __atomic_fetch_add(&g_walk[0], nr, __ATOMIC_RELAXED);
+    {   /* the groups must partition [0, na) */
+        int at = 0, ok = 1;
+        for (int k = 0; k < nr && ok; k++) {
+            if (rng[2 * k] != at || rng[2 * k + 1] <= at || rng[2 * k + 1] - at > group_size) ok = 0;
+            at = rng[2 * k + 1];
+        }
+        if (!ok || at != na) {
+            free(ks); free(ord); free(nd); free(rng);
+            return -2;
+        }
+    }
     int mq = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -690,14 +875,10 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
         int32_t* q = (int32_t*)malloc((size_t)qcap * 4);
         int32_t* cl = (int32_t*)malloc((size_t)ccap * 4);
         int32_t* pl = (int32_t*)malloc((size_t)pcap * 4);
-        int32_t sub[2 * 64];
 #pragma omp for schedule(dynamic, 16)
-        for (int g = 0; g < ng; g++) {
-          int g0 = g * group_size;
-          int g1 = g0 + group_size < na ? g0 + group_size : na;
-          int nsub = split_group(ks, ord, px, py, pz, g0, g1, 0, split_depth, sub, 0);
-          for (int sg = 0; sg < nsub; sg++) {
-            int gs = sub[2 * sg], ge = sub[2 * sg + 1];
+        for (int r = 0; r < nr; r++) {
+          {
+            int gs = rng[2 * r], ge = rng[2 * r + 1];
             double glo[3], ghi[3];
             group_box(ord, px, py, pz, gs, ge, glo, ghi);
             double lox = glo[0], loy = glo[1], loz = glo[2];
@@ -719,6 +900,7 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
                     int lim = (lifo - 8 * (levels + 2) - S) / 7;
                     if (lim < 1) lim = 1;
                     int take = S < 64 ? S : 64;
+                    __atomic_fetch_add(&g_walk[1], 1, __ATOMIC_RELAXED);
                     if (take > lim) take = lim;
                     /* serialise the chunk: process q[qt-take .. qt-1] in order,
                      * children pushed after the chunk is removed */
@@ -816,12 +998,14 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
                 if (n_cells) n_cells[i] = nc;
                 if (n_parts) n_parts[i] = np - self;
             }
+            __atomic_fetch_add(&g_walk[2], (int64_t)((nc + 63) / 64) * 64, __ATOMIC_RELAXED);
+            __atomic_fetch_add(&g_walk[3], (int64_t)((np + 63) / 64) * 64, __ATOMIC_RELAXED);
           }
         }
         free(q); free(cl); free(pl);
     }
     if (max_queue) *max_queue = mq;
-    free(ks); free(ord); free(nd);
+    free(ks); free(ord); free(nd); free(rng);
     return nn;
 }
 

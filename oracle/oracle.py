@@ -251,18 +251,20 @@ def tree_build(s, levels, leaf_cap):
 
 
 def bh_tree(s, eps, theta, levels=10, leaf_cap=16, group_size=64, shell=False,
-            split_depth=0, lifo=False, nthreads=0):
+            split_depth=-3, lifo=2048, nthreads=0):
     n = len(s["px"])
     ax, ay, az = (np.zeros(n) for _ in range(3))
     nc = np.zeros(n, dtype=np.int32)
     npart = np.zeros(n, dtype=np.int32)
     mq = C.c_int32(0)
-    lib().cmo_bh_tree(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
-                      _c(s["pz"], np.float64), _c(s["mass"], np.float64),
-                      _c(s["radius"], np.float64), _c(s["active"], np.uint8),
-                      eps, theta, int(shell), levels, leaf_cap, group_size,
-                      split_depth, int(lifo), nthreads, ax, ay, az, nc, npart,
-                      C.byref(mq))
+    rc = lib().cmo_bh_tree(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                           _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                           _c(s["radius"], np.float64), _c(s["active"], np.uint8),
+                           eps, theta, int(shell), levels, leaf_cap, group_size,
+                           split_depth, int(lifo), nthreads, ax, ay, az, nc, npart,
+                           C.byref(mq))
+    if rc < 0:
+        raise RuntimeError("oracle target groups do not partition the active bodies")
     return (ax, ay, az), {"n_cells": nc, "n_parts": npart, "max_queue": mq.value}
 
 

@@ -77,18 +77,18 @@ def test_verlet_steps_bit_exact(plummer):
         np.testing.assert_array_equal(getattr(out, k_soa)[:n], s[k_or])
 
 
-@pytest.mark.parametrize("levels,leaf,split", [(10, 16, 3), (21, 8, 3), (10, 1, 0), (21, 64, 1)])
-def test_barnes_hut_fp64_bit_exact(plummer, levels, leaf, split):
+@pytest.mark.parametrize("levels,leaf,gsize", [(10, 16, 64), (21, 8, 64), (10, 1, 32), (21, 64, 16)])
+def test_barnes_hut_fp64_bit_exact(plummer, levels, leaf, gsize):
     """Build-defined octree + group-MAC traversal in FP64 equals the oracle bit for bit,
     including the interaction counts."""
     n = 20000
     soa = _with_inactive(plummer(n, cut=0.999))
     s = soa_to_state(soa)
     be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5, morton_levels=levels,
-                                 leaf_capacity=leaf, group_split_depth=split)
+                                 leaf_capacity=leaf, group_size=gsize)
     be.compute_forces(soa, EPS)
-    ref, st = O.bh_tree(s, EPS, 0.5, levels=levels, leaf_cap=leaf, group_size=64,
-                        split_depth=split, lifo=2048, nthreads=8)
+    ref, st = O.bh_tree(s, EPS, 0.5, levels=levels, leaf_cap=leaf, group_size=gsize,
+                        split_depth=-3, lifo=2048, nthreads=8)
     _eq(_acc(soa), ref)
     gs = be.stats()
     assert gs["n_active"] == int(s["active"].sum())
@@ -105,7 +105,7 @@ def test_barnes_hut_fp32_split_vs_fp64(plummer):
     s = soa_to_state(soa)
     be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP32_SPLIT, theta=0.5)
     be.compute_forces(soa, EPS)
-    ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=3,
+    ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=-3,
                        lifo=2048, nthreads=8)
     r = _rel(_acc(soa), ref)
     assert np.median(r) <= 1e-6, np.median(r)
@@ -119,7 +119,7 @@ def test_barnes_hut_fp64_fast_vs_exact(plummer):
     s = soa_to_state(soa)
     be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64_FAST, theta=0.5)
     be.compute_forces(soa, EPS)
-    ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=3,
+    ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=-3,
                        lifo=2048, nthreads=8)
     assert _rel(_acc(soa), ref).max() <= 1e-13
 
@@ -193,3 +193,20 @@ def test_direct_sample_matches_oracle(plummer):
     be.upload(soa)
     t = np.random.default_rng(3).choice(12000, 256, replace=False).astype(np.int32)
     _eq(be.direct_sample(EPS, t), O.direct_targets(s, EPS, t))
+
+
+def test_barnes_hut_oversized_leaves_bit_exact():
+    """30-bit keys with a cluster far below one key cell: a deepest-level leaf
+    holds thousands of bodies and is split into group-sized slices."""
+    rng = np.random.default_rng(23)
+    core = rng.uniform(-1e-3, 1e-3, (3000, 3))
+    far = rng.uniform(-100, 100, (10, 3))
+    p = np.vstack([core, far])
+    soa = cmhip.BodySoA.from_arrays(pos_x=p[:, 0], pos_y=p[:, 1], pos_z=p[:, 2],
+                                    mass=rng.uniform(0.5, 2, len(p)), is_active=np.ones(len(p)))
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(len(p), precision=cmhip.PREC_FP64, theta=0.5)
+    be.compute_forces(soa, EPS)
+    ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64)
+    _eq(_acc(soa), ref)
+    assert (np.abs(soa.acc_x[:len(p)]) > 0).all()

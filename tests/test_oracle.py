@@ -295,7 +295,7 @@ def test_group_traversal_converges_to_direct():
     ref = O.direct_gk(s, 1e-3)
     errs = []
     for th in (0.2, 0.5, 0.9):
-        a, _ = O.bh_tree(s, 1e-3, th, levels=21, leaf_cap=8, group_size=64, split_depth=3,
+        a, _ = O.bh_tree(s, 1e-3, th, levels=21, leaf_cap=8, group_size=64, split_depth=-3,
                          lifo=2048)
         errs.append(np.median(rel(a, ref)))
     assert errs[0] < 1e-4 and errs[0] < errs[1] < errs[2]
@@ -303,7 +303,7 @@ def test_group_traversal_converges_to_direct():
 
 def test_group_traversal_stack_bounded():
     s = random_state(20000, seed=14, spread=3.0)
-    _, st = O.bh_tree(s, 1e-3, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=3,
+    _, st = O.bh_tree(s, 1e-3, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=-3,
                       lifo=2048)
     assert st["max_queue"] <= 2048
 
@@ -432,3 +432,29 @@ def test_compaction_order():
     out, idx = O.compact(s)
     assert list(out["px"]) == [1.0, 3.0] and list(out["radius"]) == [0.1, 0.3]
     assert list(idx) == [0, 2]
+
+
+def dense_leaf_state(n_core=3000, seed=23):
+    """A tight cluster far smaller than one 30-bit key cell of a root box
+    stretched by a few distant bodies: one deepest-level leaf holds n_core bodies."""
+    rng = np.random.default_rng(seed)
+    core = rng.uniform(-1e-3, 1e-3, (n_core, 3))
+    far = rng.uniform(-100, 100, (10, 3))
+    p = np.vstack([core, far])
+    return state(p[:, 0], p[:, 1], p[:, 2], rng.uniform(0.5, 2, len(p)))
+
+
+@pytest.mark.parametrize("policy", [3, -1, -2, -3])
+def test_groups_cover_bodies_in_oversized_leaves(policy):
+    """Every active body gets a force when a deepest-level leaf holds more
+    bodies than a group: the groups partition the bodies (the oracle raises
+    otherwise) and the result matches the direct sum."""
+    s = dense_leaf_state()
+    t = O.tree_build(s, 10, 16)
+    leaf_counts = t["count"][t["first"] < 0]
+    assert leaf_counts.max() > 64
+    a, st = O.bh_tree(s, 1e-4, 0.5, levels=10, leaf_cap=16, group_size=64,
+                      split_depth=policy)
+    ref = O.direct_gk(s, 1e-4)
+    assert rel(a, ref).max() < 1e-2
+    assert (st["n_parts"] + st["n_cells"] > 0).all()
