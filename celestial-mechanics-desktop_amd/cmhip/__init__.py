@@ -50,7 +50,10 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_get_count", "cm_compute_forces", "cm_verlet_step", "cm_energy",
            "cm_direct_sample", "cm_get_force_stats", "cm_detect_collisions",
            "cm_resolve_merges", "cm_compact", "cm_generate_plummer",
-           "cm_reset_force_stats", "cm_generate_disk")
+           "cm_reset_force_stats", "cm_generate_disk", "cm_ensemble_create",
+           "cm_ensemble_destroy", "cm_ensemble_last_error", "cm_ensemble_configure",
+           "cm_ensemble_upload", "cm_ensemble_download", "cm_ensemble_step",
+           "cm_ensemble_energy", "cm_ensemble_get_stats", "cm_generate_ensemble")
 
 # 3 g/cm^3 in Msun / AU^3 (AU = 1.496e11 m, Msun = 1.989e30 kg), and the same
 # density with radii inflated 100x (density / 100^3)
@@ -97,6 +100,16 @@ class EnergyReport(C.Structure):
                 ("active_count", C.c_int32)]
 
 
+class EnsembleStats(C.Structure):
+    _fields_ = [("threads", C.c_int32), ("tuned_threads", C.c_int32),
+                ("tuned_cost_order", C.c_int32), ("tuned_ms_per_step", C.c_float),
+                ("last_ms", C.c_float), ("last_steps", C.c_int32),
+                ("interactions_total", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EVENT_DTYPE = np.dtype([("body_a", "<i4"), ("body_b", "<i4"),
                         ("overlap_depth", "<f8"), ("distance", "<f8"),
                         ("normal_x", "<f8"), ("normal_y", "<f8"),
@@ -136,6 +149,17 @@ def load_library():
                                  C.POINTER(SoAPtrs)], i32),
         "cm_generate_disk": ([i32, C.c_uint64, f64, f64, f64, f64, f64, f64, f64, f64,
                               C.POINTER(SoAPtrs)], i32),
+        "cm_ensemble_create": ([i32, vp, i32, C.POINTER(vp)], i32),
+        "cm_ensemble_destroy": ([vp], i32),
+        "cm_ensemble_last_error": ([vp], C.c_char_p),
+        "cm_ensemble_configure": ([vp, i32, i32, i32], i32),
+        "cm_ensemble_upload": ([vp, C.POINTER(SoAPtrs)], i32),
+        "cm_ensemble_download": ([vp, C.POINTER(SoAPtrs)], i32),
+        "cm_ensemble_step": ([vp, f64, f64, i32], i32),
+        "cm_ensemble_energy": ([vp, f64, vp], i32),
+        "cm_ensemble_get_stats": ([vp, C.POINTER(EnsembleStats)], i32),
+        "cm_generate_ensemble": ([i32, C.c_uint64, i32, i32, f64, f64, f64, f64, f64, vp,
+                                  C.POINTER(SoAPtrs)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -411,3 +435,82 @@ class CollisionResolver:
     def resolve(self, backend):
         self.last_collision_count = backend.resolve_merges()
         return self.last_collision_count
+
+
+def generate_ensemble(n_systems, seed=42, n_min=8, n_max=2048, m_min=1e-7, m_max=1e-3,
+                      a_min=0.1, a_max=30.0, sigma_e=0.05):
+    """Planetary systems for the batched ensemble (see cm_generate_ensemble).
+    Returns (BodySoA of all bodies, offsets)."""
+    L = load_library()
+    offsets = np.zeros(n_systems + 1, dtype=np.int32)
+    rc = L.cm_generate_ensemble(n_systems, C.c_uint64(seed), n_min, n_max, m_min, m_max,
+                                a_min, a_max, sigma_e, offsets.ctypes.data, None)
+    if rc != CM_OK:
+        raise CmError(rc, "cm_generate_ensemble")
+    soa = BodySoA(int(offsets[-1]))
+    rc = L.cm_generate_ensemble(n_systems, C.c_uint64(seed), n_min, n_max, m_min, m_max,
+                                a_min, a_max, sigma_e, offsets.ctypes.data,
+                                C.byref(soa.ptrs()))
+    if rc != CM_OK:
+        raise CmError(rc, "cm_generate_ensemble")
+    soa.count = int(offsets[-1])
+    return soa, offsets
+
+
+class HipEnsemble:
+    """Many independent systems stepped together (cm_ensemble_*): each system
+    is a BodySoA slice advanced by SoAVerletIntegrator.Step with the
+    CpuParallelBackend direct sweep."""
+
+    def __init__(self, offsets, precision=PREC_FP32_SPLIT, threads=0, steps_per_launch=100,
+                 device=0):
+        L = load_library()
+        self._L = L
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.n_systems = len(self.offsets) - 1
+        h = C.c_void_p()
+        rc = L.cm_ensemble_create(self.n_systems, self.offsets.ctypes.data, device, C.byref(h))
+        if rc != CM_OK:
+            raise CmError(rc, "cm_ensemble_create")
+        self._h = h
+        self._check(L.cm_ensemble_configure(h, precision, threads, steps_per_launch),
+                    "cm_ensemble_configure")
+
+    def _check(self, rc, what):
+        if rc != CM_OK:
+            msg = self._L.cm_ensemble_last_error(self._h)
+            raise CmError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def upload(self, bodies):
+        self._check(self._L.cm_ensemble_upload(self._h, C.byref(bodies.ptrs())),
+                    "cm_ensemble_upload")
+
+    def download(self, bodies):
+        self._check(self._L.cm_ensemble_download(self._h, C.byref(bodies.ptrs())),
+                    "cm_ensemble_download")
+
+    def step(self, softening, dt, n_steps):
+        self._check(self._L.cm_ensemble_step(self._h, float(softening), float(dt), int(n_steps)),
+                    "cm_ensemble_step")
+
+    def energy(self, softening):
+        out = np.zeros(self.n_systems)
+        self._check(self._L.cm_ensemble_energy(self._h, float(softening), out.ctypes.data),
+                    "cm_ensemble_energy")
+        return out
+
+    def stats(self):
+        s = EnsembleStats()
+        self._check(self._L.cm_ensemble_get_stats(self._h, C.byref(s)), "cm_ensemble_get_stats")
+        return s.as_dict()
+
+    def close(self):
+        if self._h:
+            self._L.cm_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -190,3 +190,57 @@ extern "C" int32_t cm_generate_disk(int32_t n, uint64_t seed, double star_radius
     com_shift(n, out);
     return CM_OK;
 }
+
+extern "C" int32_t cm_generate_ensemble(int32_t n_systems, uint64_t seed, int32_t n_min,
+                                        int32_t n_max, double m_min, double m_max,
+                                        double a_min, double a_max, double sigma_e,
+                                        int32_t* offsets, const cm_soa_ptrs* out) {
+    if (n_systems < 1 || n_min < 1 || n_max < n_min || !offsets || !(m_min > 0.0) ||
+        !(m_max >= m_min) || !(a_min > 0.0) || !(a_max >= a_min))
+        return CM_ERR_INVALID_ARGUMENT;
+    // sizes from their own stream so that the offsets can be had alone
+    SplitMix64 sizes(seed ^ 0x5EEDull);
+    offsets[0] = 0;
+    for (int32_t s = 0; s < n_systems; s++) {
+        int32_t span = n_max - n_min + 1;
+        int32_t nb = n_min + (int32_t)(sizes.next() % (uint64_t)span);
+        offsets[s + 1] = offsets[s] + nb;
+    }
+    if (!out) return CM_OK;
+    if (!out->pos_x || !out->pos_y || !out->pos_z || !out->vel_x || !out->vel_y ||
+        !out->vel_z || !out->mass)
+        return CM_ERR_INVALID_ARGUMENT;
+    SplitMix64 rng(seed);
+    const double la = std::log(a_min), lb = std::log(a_max);
+    const double lm0 = std::log(m_min), lm1 = std::log(m_max);
+    for (int32_t s = 0; s < n_systems; s++) {
+        int32_t b0 = offsets[s], b1 = offsets[s + 1];
+        out->pos_x[b0] = out->pos_y[b0] = out->pos_z[b0] = 0.0;
+        out->vel_x[b0] = out->vel_y[b0] = out->vel_z[b0] = 0.0;
+        out->mass[b0] = 1.0;
+        for (int32_t i = b0 + 1; i < b1; i++) {
+            double a = std::exp(la + (lb - la) * rng.uniform());
+            double e = std::min(rayleigh(rng, sigma_e), 0.9);
+            double inc = std::acos(2.0 * rng.uniform() - 1.0);
+            double node = 2.0 * kPi * rng.uniform();
+            double peri = 2.0 * kPi * rng.uniform();
+            double mean = 2.0 * kPi * rng.uniform();
+            double m = std::exp(lm0 + (lm1 - lm0) * rng.uniform());
+            double r[3], v[3];
+            kepler_to_cartesian(1.0 + m, a, e, inc, node, peri, mean, r, v);
+            out->pos_x[i] = r[0]; out->pos_y[i] = r[1]; out->pos_z[i] = r[2];
+            out->vel_x[i] = v[0]; out->vel_y[i] = v[1]; out->vel_z[i] = v[2];
+            out->mass[i] = m;
+        }
+        cm_soa_ptrs sub = *out;
+        sub.pos_x += b0; sub.pos_y += b0; sub.pos_z += b0;
+        sub.vel_x += b0; sub.vel_y += b0; sub.vel_z += b0;
+        sub.mass += b0;
+        com_shift(b1 - b0, &sub);
+        for (int32_t i = b0; i < b1; i++) {
+            if (out->radius) out->radius[i] = 0.0;
+            if (out->is_active) out->is_active[i] = 1;
+        }
+    }
+    return CM_OK;
+}

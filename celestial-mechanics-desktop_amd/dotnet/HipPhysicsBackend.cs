@@ -1,0 +1,120 @@
+// HipPhysicsBackend.cs -- the reference-side binding of libcmhip.so.
+//
+// Drop-in IPhysicsComputeBackend + IGravityModelAwareBackend for
+// src/CelestialMechanics.Physics (mfscpayload-690/celestial-mechanics-desktop),
+// in the slot CudaPhysicsBackend reserves (Solvers/CudaPhysicsBackend.cs:35-48).
+// Not compiled in this repository (no .NET toolchain here); it is the file a
+// maintainer adds to src/CelestialMechanics.Physics/Solvers/.  Requires
+// <AllowUnsafeBlocks>true</AllowUnsafeBlocks> in the project.
+using System;
+using System.Runtime.InteropServices;
+using CelestialMechanics.Physics.SoA;
+
+namespace CelestialMechanics.Physics.Solvers;
+
+internal static unsafe class CmHip
+{
+    private const string Lib = "cmhip";   // libcmhip.so / cmhip.dll
+
+    [StructLayout(LayoutKind.Sequential)]
+    public struct BackendConfig
+    {
+        public int Kind;            // 0 pairwise (CpuSingleThreadBackend), 1 sweep (CpuParallelBackend), 2 Barnes-Hut
+        public int Precision;       // 0 FP64, 1 FP64 fast rsqrt, 2 FP32 split
+        public int ShellTheorem;
+        public int MortonLevels;    // 10 -> 30-bit keys
+        public double Theta;
+        public int LeafCapacity;
+        public int GroupSize;
+    }
+
+    [StructLayout(LayoutKind.Sequential)]
+    public struct SoAPtrs
+    {
+        public double* PosX, PosY, PosZ, VelX, VelY, VelZ, AccX, AccY, AccZ;
+        public double* OldAccX, OldAccY, OldAccZ, Mass, Radius, Density;
+        public byte* IsActive, IsCollidable;   // bool[] travels pinned as byte*
+        public int* BodyTypeIndex;
+    }
+
+    [DllImport(Lib)] public static extern int cm_create(int capacity, int device, out IntPtr engine);
+    [DllImport(Lib)] public static extern int cm_destroy(IntPtr engine);
+    [DllImport(Lib)] public static extern IntPtr cm_last_error(IntPtr engine);
+    [DllImport(Lib)] public static extern int cm_default_config(ref BackendConfig cfg);
+    [DllImport(Lib)] public static extern int cm_configure(IntPtr engine, ref BackendConfig cfg);
+    [DllImport(Lib)] public static extern int cm_upload_soa(IntPtr engine, ref SoAPtrs src, int count);
+    [DllImport(Lib)] public static extern int cm_download_soa(IntPtr engine, ref SoAPtrs dst, int count);
+    [DllImport(Lib)] public static extern int cm_compute_forces(IntPtr engine, double softening);
+    [DllImport(Lib)] public static extern int cm_verlet_step(IntPtr engine, double softening, double dt, int nSteps);
+
+    public static void Check(IntPtr engine, int rc, string what)
+    {
+        if (rc == 0) return;
+        string msg = Marshal.PtrToStringAnsi(cm_last_error(engine)) ?? "";
+        throw new InvalidOperationException($"{what} failed ({rc}): {msg}");
+    }
+}
+
+/// <summary>
+/// GPU (MI355X) backend: uploads the SoA state, computes accelerations on the
+/// device (Barnes-Hut by default) and writes AccX/Y/Z for every index below
+/// Count, inactive bodies getting 0 -- the IPhysicsComputeBackend contract.
+/// </summary>
+public sealed unsafe class HipPhysicsBackend : IPhysicsComputeBackend, IGravityModelAwareBackend, IDisposable
+{
+    private IntPtr _engine;
+    private CmHip.BackendConfig _cfg;
+    private readonly int _capacity;
+
+    public HipPhysicsBackend(int capacity, double theta = 0.5, int precision = 2)
+    {
+        _capacity = capacity;
+        CmHip.Check(IntPtr.Zero, CmHip.cm_create(capacity, 0, out _engine), "cm_create");
+        CmHip.cm_default_config(ref _cfg);
+        _cfg.Theta = theta;
+        _cfg.Precision = precision;
+        CmHip.Check(_engine, CmHip.cm_configure(_engine, ref _cfg), "cm_configure");
+    }
+
+    public bool EnableShellTheorem
+    {
+        get => _cfg.ShellTheorem != 0;
+        set { _cfg.ShellTheorem = value ? 1 : 0; CmHip.Check(_engine, CmHip.cm_configure(_engine, ref _cfg), "cm_configure"); }
+    }
+
+    public double Theta
+    {
+        get => _cfg.Theta;
+        set { _cfg.Theta = value; CmHip.Check(_engine, CmHip.cm_configure(_engine, ref _cfg), "cm_configure"); }
+    }
+
+    public void ComputeForces(BodySoA bodies, double softening)
+    {
+        if (bodies.Count > _capacity) throw new InvalidOperationException("body count exceeds backend capacity");
+        fixed (double* px = bodies.PosX, py = bodies.PosY, pz = bodies.PosZ,
+               vx = bodies.VelX, vy = bodies.VelY, vz = bodies.VelZ,
+               ax = bodies.AccX, ay = bodies.AccY, az = bodies.AccZ,
+               ox = bodies.OldAccX, oy = bodies.OldAccY, oz = bodies.OldAccZ,
+               m = bodies.Mass, r = bodies.Radius, d = bodies.Density)
+        fixed (bool* act = bodies.IsActive, col = bodies.IsCollidable)
+        fixed (int* type = bodies.BodyTypeIndex)
+        {
+            var p = new CmHip.SoAPtrs
+            {
+                PosX = px, PosY = py, PosZ = pz, VelX = vx, VelY = vy, VelZ = vz,
+                AccX = ax, AccY = ay, AccZ = az, OldAccX = ox, OldAccY = oy, OldAccZ = oz,
+                Mass = m, Radius = r, Density = d,
+                IsActive = (byte*)act, IsCollidable = (byte*)col, BodyTypeIndex = type,
+            };
+            CmHip.Check(_engine, CmHip.cm_upload_soa(_engine, ref p, bodies.Count), "cm_upload_soa");
+            CmHip.Check(_engine, CmHip.cm_compute_forces(_engine, softening), "cm_compute_forces");
+            var outp = new CmHip.SoAPtrs { AccX = ax, AccY = ay, AccZ = az };
+            CmHip.Check(_engine, CmHip.cm_download_soa(_engine, ref outp, bodies.Count), "cm_download_soa");
+        }
+    }
+
+    public void Dispose()
+    {
+        if (_engine != IntPtr.Zero) { CmHip.cm_destroy(_engine); _engine = IntPtr.Zero; }
+    }
+}

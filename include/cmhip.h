@@ -187,7 +187,51 @@ int32_t cm_resolve_merges(void* engine, int32_t* n_merged);
  * receives, for each survivor, its index before compaction. */
 int32_t cm_compact(void* engine, int32_t* new_count, int32_t* old_index);
 
+/* ---- batched ensemble of independent systems ---------------------------- */
+/* Many independent systems (each <= 2048 bodies, concatenated; system s owns
+ * bodies offsets[s] .. offsets[s+1]-1) stepped with SoAVerletIntegrator.Step
+ * (Integrators/SoAVerletIntegrator.cs:48-115) and CpuParallelBackend's
+ * direct sweep (Solvers/CpuParallelBackend.cs:99-136) per system.  The
+ * reference steps one system per NBodySolver; this batches thousands in one
+ * persistent launch replayed from a hipGraph. */
+typedef struct {
+    int32_t threads;             /* workgroup size in use */
+    int32_t tuned_threads;       /* autotuner choice (0: not tuned) */
+    int32_t tuned_cost_order;    /* 1: systems ordered by N^2 descending */
+    float tuned_ms_per_step;
+    float last_ms;               /* last cm_ensemble_step call, HIP events */
+    int32_t last_steps;
+    int64_t interactions_total;  /* pair terms evaluated since creation */
+} cm_ensemble_stats;
+
+int32_t cm_ensemble_create(int32_t n_systems, const int32_t* offsets,
+                           int32_t device, void** out_ensemble);
+int32_t cm_ensemble_destroy(void* ensemble);
+const char* cm_ensemble_last_error(void* ensemble);
+/* precision: CM_PREC_FP64 or CM_PREC_FP32_SPLIT; threads: 0 (autotune on the
+ * next step), 256, 512 or 1024; steps_per_launch: steps per captured launch */
+int32_t cm_ensemble_configure(void* ensemble, int32_t precision, int32_t threads,
+                              int32_t steps_per_launch);
+int32_t cm_ensemble_upload(void* ensemble, const cm_soa_ptrs* src);
+int32_t cm_ensemble_download(void* ensemble, const cm_soa_ptrs* dst);
+int32_t cm_ensemble_step(void* ensemble, double softening, double dt,
+                         int32_t n_steps);
+/* per-system total energy KE + PE (EnergyCalculator quantities), FP64 with
+ * compensated sums; energy has n_systems entries */
+int32_t cm_ensemble_energy(void* ensemble, double softening, double* energy);
+int32_t cm_ensemble_get_stats(void* ensemble, cm_ensemble_stats* out);
+
 /* ---- synthetic scenarios (host-side, deterministic from a seed) ----------- */
+/* Planetary systems for the ensemble: system sizes uniform in [n_min, n_max];
+ * each is a 1 Msun star plus planets with log-uniform masses in
+ * [m_min, m_max], log-uniform semi-major axes in [a_min, a_max], Rayleigh
+ * eccentricity (sigma_e), isotropic orientation, centre-of-mass frame.
+ * Call with out == NULL to get offsets (n_systems + 1 entries) only; then
+ * with arrays of offsets[n_systems] bodies. */
+int32_t cm_generate_ensemble(int32_t n_systems, uint64_t seed, int32_t n_min,
+                             int32_t n_max, double m_min, double m_max,
+                             double a_min, double a_max, double sigma_e,
+                             int32_t* offsets, const cm_soa_ptrs* out);
 /* PlummerSphere.Generate (ScenarioGenerators/PlummerSphere.cs:41-116) with a
  * SplitMix64 stream; mass_fraction_cut < 1 truncates the sphere at the radius
  * enclosing that mass fraction; com_frame subtracts the centre-of-mass
