@@ -53,7 +53,11 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_reset_force_stats", "cm_generate_disk", "cm_ensemble_create",
            "cm_ensemble_destroy", "cm_ensemble_last_error", "cm_ensemble_configure",
            "cm_ensemble_upload", "cm_ensemble_download", "cm_ensemble_step",
-           "cm_ensemble_energy", "cm_ensemble_get_stats", "cm_generate_ensemble")
+           "cm_ensemble_energy", "cm_ensemble_get_stats", "cm_generate_ensemble",
+           "cm_mlp_init", "cm_mlp_set_params", "cm_mlp_get_params", "cm_mlp_forward",
+           "cm_mlp_gradient", "cm_mlp_collect", "cm_mlp_get_pairs", "cm_mlp_train")
+
+MLP_IN, MLP_OUT, MLP_USED, MLP_PARAMS = 16, 16, 8, 10448
 
 # 3 g/cm^3 in Msun / AU^3 (AU = 1.496e11 m, Msun = 1.989e30 kg), and the same
 # density with radii inflated 100x (density / 100^3)
@@ -71,7 +75,7 @@ class BackendConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("precision", C.c_int32),
                 ("shell_theorem", C.c_int32), ("morton_levels", C.c_int32),
                 ("theta", C.c_double), ("leaf_capacity", C.c_int32),
-                ("group_size", C.c_int32)]
+                ("group_size", C.c_int32), ("learned_correction", C.c_int32)]
 
 
 class SoAPtrs(C.Structure):
@@ -160,6 +164,14 @@ def load_library():
         "cm_ensemble_get_stats": ([vp, C.POINTER(EnsembleStats)], i32),
         "cm_generate_ensemble": ([i32, C.c_uint64, i32, i32, f64, f64, f64, f64, f64, vp,
                                   C.POINTER(SoAPtrs)], i32),
+        "cm_mlp_init": ([vp, C.c_uint64], i32),
+        "cm_mlp_set_params": ([vp, vp, i32], i32),
+        "cm_mlp_get_params": ([vp, vp, i32], i32),
+        "cm_mlp_forward": ([vp, vp, i32, vp], i32),
+        "cm_mlp_gradient": ([vp, vp, vp, i32, vp, C.POINTER(C.c_float)], i32),
+        "cm_mlp_collect": ([vp, f64, i32, C.c_uint64, i32, C.POINTER(i32)], i32),
+        "cm_mlp_get_pairs": ([vp, vp, vp, vp, vp, i32], i32),
+        "cm_mlp_train": ([vp, i32, i32, C.c_float, C.c_uint64, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -234,6 +246,30 @@ def generate_plummer(count, total_mass=1.0, scale_length=1.0, seed=42,
     return soa
 
 
+def generate_merger(count, seed=42, separation=10.0, speed_fraction=0.3, mass_each=0.5,
+                    scale_length=1.0, mass_fraction_cut=0.999, capacity=None):
+    """Two equal Plummer spheres (count / 2 bodies each, SplitMix64 seeds seed
+    and seed + 1) with centres `separation` apart on the x axis, approaching
+    head-on with relative speed speed_fraction x the escape speed
+    sqrt(2 G M_total / separation); centre-of-mass frame.  No reference
+    generator exists for it (a synthetic merger workload)."""
+    h1 = count // 2
+    h2 = count - h1
+    a = generate_plummer(h1, mass_each, scale_length, seed, mass_fraction_cut)
+    b = generate_plummer(h2, mass_each, scale_length, seed + 1, mass_fraction_cut)
+    v = speed_fraction * np.sqrt(2.0 * 2.0 * mass_each / separation)
+    soa = BodySoA(capacity or max(count, 1))
+    for f in F64_FIELDS + U8_FIELDS + I32_FIELDS:
+        getattr(soa, f)[:h1] = getattr(a, f)[:h1]
+        getattr(soa, f)[h1:count] = getattr(b, f)[:h2]
+    soa.pos_x[:h1] -= 0.5 * separation
+    soa.pos_x[h1:count] += 0.5 * separation
+    soa.vel_x[:h1] += 0.5 * v
+    soa.vel_x[h1:count] -= 0.5 * v
+    soa.count = count
+    return soa
+
+
 def generate_disk(count, seed=42, star_radius=0.005, a_min=0.5, a_max=5.0, sigma_e=0.02,
                   sigma_i=0.01, m_min=1e-9, m_max=1e-6, density=ROCK_DENSITY_INFLATED,
                   capacity=None):
@@ -258,7 +294,8 @@ class HipPhysicsBackend:
 
     def __init__(self, capacity, kind=BACKEND_BARNES_HUT, precision=PREC_FP64,
                  theta=0.5, morton_levels=10, leaf_capacity=16,
-                 group_size=64, enable_shell_theorem=False, device=0):
+                 group_size=64, enable_shell_theorem=False, learned_correction=False,
+                 device=0):
         L = load_library()
         h = C.c_void_p()
         rc = L.cm_create(int(capacity), int(device), C.byref(h))
@@ -276,6 +313,7 @@ class HipPhysicsBackend:
         self.config.leaf_capacity = leaf_capacity
         self.config.group_size = group_size
         self.config.shell_theorem = int(enable_shell_theorem)
+        self.config.learned_correction = int(learned_correction)
         self._apply()
 
     # -- configuration ----------------------------------------------------
@@ -387,6 +425,60 @@ class HipPhysicsBackend:
         n = C.c_int32()
         self._check(self._L.cm_compact(self._h, C.byref(n), old.ctypes.data), "cm_compact")
         return n.value, old[:n.value].copy()
+
+    # -- learned far-field correction ----------------------------------------
+    def mlp_init(self, seed=1):
+        self._check(self._L.cm_mlp_init(self._h, int(seed)), "cm_mlp_init")
+
+    def mlp_params(self):
+        p = np.zeros(MLP_PARAMS, dtype=np.float32)
+        self._check(self._L.cm_mlp_get_params(self._h, p.ctypes.data, MLP_PARAMS),
+                    "cm_mlp_get_params")
+        return p
+
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self._L.cm_mlp_set_params(self._h, p.ctypes.data, len(p)),
+                    "cm_mlp_set_params")
+
+    def mlp_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        y = np.zeros((len(x), MLP_OUT), dtype=np.float32)
+        self._check(self._L.cm_mlp_forward(self._h, x.ctypes.data, len(x), y.ctypes.data),
+                    "cm_mlp_forward")
+        return y
+
+    def mlp_gradient(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, MLP_IN)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1, MLP_USED)
+        g = np.zeros(MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        self._check(self._L.cm_mlp_gradient(self._h, x.ctypes.data, t.ctypes.data, len(x),
+                                             g.ctypes.data, C.byref(loss)), "cm_mlp_gradient")
+        return g, loss.value
+
+    def mlp_collect(self, softening, max_groups=256, seed=7, max_pairs=1 << 20):
+        n = C.c_int32()
+        self._check(self._L.cm_mlp_collect(self._h, float(softening), int(max_groups),
+                                           int(seed), int(max_pairs), C.byref(n)),
+                    "cm_mlp_collect")
+        return n.value
+
+    def mlp_pairs(self, n):
+        x = np.zeros((n, MLP_IN), dtype=np.float32)
+        t = np.zeros((n, MLP_USED), dtype=np.float32)
+        gn = np.zeros((n, 4), dtype=np.int32)
+        c = np.zeros((n, 3))
+        self._check(self._L.cm_mlp_get_pairs(self._h, x.ctypes.data, t.ctypes.data,
+                                             gn.ctypes.data, c.ctypes.data, n),
+                    "cm_mlp_get_pairs")
+        return x, t, gn, c
+
+    def mlp_train(self, epochs=20, batch=1024, lr=1e-3, seed=11):
+        losses = np.zeros(epochs, dtype=np.float32)
+        self._check(self._L.cm_mlp_train(self._h, int(epochs), int(batch), float(lr),
+                                         int(seed), losses.ctypes.data), "cm_mlp_train")
+        return losses
 
     def close(self):
         if self._h:

@@ -18,7 +18,10 @@ constexpr int kMaxLevels = 21;          // 63-bit keys
 constexpr int kStackCap = 2048;         // per-wave LDS traversal stack (ints)
 constexpr int kSortTile = 2048;         // keys per radix-sort tile
 constexpr int kSortThreads = 256;
-constexpr int kBuildTile = 256;         // nodes per tree-build tile
+constexpr int kTreeTile = 1024;         // bodies per tree-build tile
+
+// rows of per-tile node counts the tree build keeps per level
+inline int tree_tiles(int capacity) { return (capacity + kTreeTile - 1) / kTreeTile + 1; }
 
 // SoA field order of BodySoA (SoA/BodySoA.cs:42-66).
 enum Field : int {
@@ -34,7 +37,6 @@ struct TreeHeader {
     int32_t overflow;
     int32_t n_levels_used;
     int32_t level_start[kMaxLevels + 3];
-    uint32_t tile_counter[kMaxLevels + 2];
     unsigned long long n_cell_int;
     unsigned long long n_body_int;
     unsigned long long acc_int;   // cell + body terms since the last stats reset
@@ -65,6 +67,7 @@ struct Engine {
     int32_t* vals_b = nullptr;
     uint32_t* hist = nullptr;          // 256 x sort tiles
     double* sorted = nullptr;          // 5 x capacity: x, y, z, m, radius
+    double4* pm4 = nullptr;            // per body (x, y, z, m), written with the keys
     double* bbox_part = nullptr;       // per-block min/max partials
     int32_t* bbox_cnt = nullptr;
     TreeHeader* hdr = nullptr;
@@ -74,8 +77,15 @@ struct Engine {
     double4* nd_cm = nullptr;          // com x, y, z, mass
     double* nd_sum = nullptr;          // 3 x node_cap: sum m*x, m*y, m*z
     double* nd_box = nullptr;          // 6 x node_cap: body bounding box lo, hi
-    unsigned long long* lookback = nullptr;  // (levels + 1) x build tiles
+    int32_t* nd_parent = nullptr;      // parent node of every node but the root
+    uchar2* tree_dl = nullptr;         // per sorted body: shared digits with the
+                                       // previous key, deepest level it opens
+    uint32_t* tree_tcount = nullptr;   // (kMaxLevels + 1) x tree_tiles: nodes per tile
     int2* groups = nullptr;            // target groups: sorted body range [x, y)
+    float* nd_quad = nullptr;          // 6 x node_cap second moments (learned correction)
+
+    // learned far-field correction (k_mlp.hip)
+    void* mlp = nullptr;
 
     // collisions / compaction scratch
     void* coll = nullptr;
@@ -114,6 +124,12 @@ int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne)
 int collide_resolve(Engine& e, int32_t* merged);
 int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index);
 void collide_free(Engine& e);
+int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out);
+int form_groups(Engine& e);
+void mlp_free(Engine& e);
+bool mlp_ready(const Engine& e);
+const uint16_t* mlp_weights(const Engine& e);   // bf16 [out][in] per layer
+const float* mlp_params(const Engine& e);       // FP32 master (biases)
 
 void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
                       uint64_t** k, int32_t** v, uint64_t** kt, int32_t** vt,

@@ -150,11 +150,13 @@ static void free_all(Engine& e) {
     for (int k = 0; k < NF64; k++) if (e.f[k]) (void)hipFree(e.f[k]);
     void* ptrs[] = {e.act, e.col, e.type, e.keys_a, e.keys_b, e.vals_a,
                     e.vals_b, e.hist, e.sorted, e.bbox_part, e.bbox_cnt, e.hdr,
-                    e.nd_info, e.nd_cm, e.nd_sum, e.lookback, e.red, e.sample,
+                    e.nd_info, e.nd_cm, e.nd_sum, e.nd_parent, e.tree_dl, e.tree_tcount, e.pm4, e.nd_quad,
+                    e.red, e.sample,
                     e.groups, e.nd_box};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     if (e.hdr_host) (void)hipHostFree(e.hdr_host);
     collide_free(e);
+    mlp_free(e);
     for (auto& ev : e.ev) if (ev) (void)hipEventDestroy(ev);
     if (e.wev) {
         for (int k = 0; k < 3 * kWindow; k++) if (e.wev[k]) (void)hipEventDestroy(e.wev[k]);
@@ -180,6 +182,7 @@ int32_t cm_default_config(cm_backend_config* cfg) {
     cfg->theta = 0.5;   // BarnesHutBackend.Theta default (BarnesHutBackend.cs:75)
     cfg->leaf_capacity = 16;
     cfg->group_size = 64;
+    cfg->learned_correction = 0;
     return CM_OK;
 }
 
@@ -220,6 +223,7 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     size_t tiles = (n + cm::kSortTile - 1) / cm::kSortTile;
     alloc((void**)&e->hist, 256 * (tiles + 1) * 4);
     alloc((void**)&e->sorted, 5 * n * 8);
+    alloc((void**)&e->pm4, n * sizeof(double4));
     alloc((void**)&e->bbox_part, 6 * 1024 * 8);
     alloc((void**)&e->bbox_cnt, 1024 * 4);
     alloc((void**)&e->hdr, sizeof(cm::TreeHeader));
@@ -228,8 +232,10 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     alloc((void**)&e->nd_cm, (size_t)e->node_cap * sizeof(double4));
     alloc((void**)&e->nd_sum, (size_t)e->node_cap * 3 * 8);
     alloc((void**)&e->nd_box, (size_t)e->node_cap * 6 * 8);
-    size_t btiles = ((size_t)e->node_cap + cm::kBuildTile - 1) / cm::kBuildTile;
-    alloc((void**)&e->lookback, (cm::kMaxLevels + 1) * btiles * 8);
+    alloc((void**)&e->nd_parent, (size_t)e->node_cap * 4);
+    alloc((void**)&e->nd_quad, (size_t)e->node_cap * 6 * 4);
+    alloc((void**)&e->tree_dl, n * sizeof(uchar2));
+    alloc((void**)&e->tree_tcount, (size_t)(cm::kMaxLevels + 1) * cm::tree_tiles((int)n) * 4);
     alloc((void**)&e->groups, n * sizeof(int2));
     alloc((void**)&e->red, 64 * 1024 * 8);
     alloc((void**)&e->sample, 8);
@@ -285,6 +291,8 @@ int32_t cm_configure(void* h, const cm_backend_config* cfg) {
                              "group_size must be in 1..64");
     if (!(cfg->theta >= 0.0) || !(cfg->theta < 1e6))
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "theta must be finite, >= 0");
+    if (cfg->learned_correction != 0 && cfg->learned_correction != 1)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "learned_correction must be 0 or 1");
     e->cfg = *cfg;
     return CM_OK;
 }

@@ -33,11 +33,10 @@
 
 #include <algorithm>
 
+#include "bh_device.h"
 #include "cm_internal.h"
 
 namespace cm {
-
-int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out);
 
 // ------------------------------------------------------------ grouping ----
 __device__ __forceinline__ double diag2_of(const double* lo, const double* hi) {
@@ -75,6 +74,9 @@ __device__ bool box_split_nodes(const double* lo1, const double* hi1,
 __global__ void __launch_bounds__(256)
 k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
          const double* __restrict__ nbox, int gsize, int2* __restrict__ groups) {
+    // a tree that outgrew the node pool is incomplete: no groups, no
+    // traversal (the evaluation reports CM_ERR_POOL_EXHAUSTED)
+    if (hdr->overflow) return;
     const int nn = hdr->n_nodes;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < nn; k += gridDim.x * 256) {
         int4 nd = info[k];
@@ -139,6 +141,10 @@ struct TravArgs {
     int levels;
     int shell;
     unsigned long long* counters;   // [0] cell, [1] body, [2] accumulated terms
+    // learned far-field correction (k_traverse<FP32_SPLIT, true> only)
+    const uint16_t* mlp_w;          // bf16 weights [out][in] per layer
+    const float* mlp_p;             // FP32 parameters (biases)
+    const float* quad;              // 6 second moments per node
 };
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
@@ -210,7 +216,7 @@ struct Staging { double4 v[kWave]; double r[kWave]; int id[kWave]; };
 template <>
 struct Staging<CM_PREC_FP32_SPLIT> { float4 v[kWave]; float r[kWave]; int id[kWave]; };
 
-template <int PREC>
+template <int PREC, bool LEARNED>
 __global__ void __launch_bounds__(kWave)
 k_traverse(TravArgs a) {
     __shared__ int s_stack[kStackCap];
@@ -220,6 +226,9 @@ k_traverse(TravArgs a) {
     __shared__ int s_lstart[kWave];
     __shared__ int s_g;
     __shared__ Staging<PREC> st;
+    // MLP inputs, bf16 [cell][16]; read as 8-byte operands
+    __shared__ __align__(16) uint16_t s_mx[LEARNED ? kWave * kMlpIn : 4];
+    __shared__ float s_my[LEARNED ? kWave * kMlpUsed : 1];   // MLP outputs per cell
 
     const int lane = lane_id();
     const int ngroups = a.hdr->n_groups;
@@ -260,6 +269,103 @@ k_traverse(TravArgs a) {
         const float txf = (float)(xi - cxg), tyf = (float)(yi - cyg), tzf = (float)(zi - czg);
         double acx = 0.0, acy = 0.0, acz = 0.0;   // monopole sums
         double apx = 0.0, apy = 0.0, apz = 0.0;   // body sums (FP32: folded partials)
+        // learned correction summed over this lane's cells (lane = list
+        // entry): residual at the group centre and its gradient (xx, yy, zz,
+        // xy, xz, yz), in the original axes
+        float lres[3] = {0.0f, 0.0f, 0.0f};
+        float lgrad[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+
+        // MLP over the staged cells (lane j = cell j < cnt), one 16-cell tile
+        // at a time; hidden activations stay in registers: the transposed
+        // layer Z^T = W X^T leaves lane l with Z[cell l % 16][4 (l / 16) + r],
+        // which is exactly the B operand the next layer needs
+        auto learned_flush = [&](int cnt) {
+          if constexpr (LEARNED && PREC == CM_PREC_FP32_SPLIT) {
+            const int lr = lane & 15, lg = lane >> 4;
+            float sres = 0.0f, sgrad = 0.0f;
+            float x[kMlpIn], f[9];
+            for (int k = 0; k < kMlpIn; k++) x[k] = 0.0f;
+            for (int k = 0; k < 9; k++) f[k] = 0.0f;
+            if (lane < cnt) {
+                const int node = s_cell[lane];
+                const int4 ni = a.info[node];
+                if (ni.w >= 2) {   // one-body cells have no residual
+                    const float4 v = st.v[lane];
+                    const float hl = (float)ldexp(h, -(ni.y >> 8));
+                    mlp_inputs(v.x, v.y, v.z, v.w, hl, a.quad + 6 * (size_t)node, x, sres,
+                               sgrad, f);
+                }
+            }
+            for (int k = 0; k < kMlpIn; k++) s_mx[lane * kMlpIn + k] = f2bf(x[k]);
+            __syncthreads();
+            const uint16_t* W = a.mlp_w;
+            const float* P = a.mlp_p;
+            for (int tile = 0; tile < 4; tile++) {
+                if (tile * 16 >= cnt) break;
+                v4s hin[4], hout[4];
+                // layer 1 (K = 16)
+                {
+                    v4s b = *(const v4s*)&s_mx[(tile * 16 + lr) * kMlpIn + 4 * lg];
+                    for (int ot = 0; ot < 4; ot++) {
+                        v4s w = *(const v4s*)&W[kBW1 + (ot * 16 + lr) * kMlpIn + 4 * lg];
+                        v4f z = mfma16(w, b, v4f{0.0f, 0.0f, 0.0f, 0.0f});
+                        for (int r = 0; r < 4; r++) {
+                            float t = z[r] + P[kPB1 + ot * 16 + 4 * lg + r];
+                            hin[ot][r] = (short)f2bf(t > 0.0f ? t : 0.0f);
+                        }
+                    }
+                }
+                // layers 2 and 3 (K = 64)
+                for (int layer = 0; layer < 2; layer++) {
+                    const int wo = layer == 0 ? kBW2 : kBW3;
+                    const int bo = layer == 0 ? kPB2 : kPB3;
+                    for (int ot = 0; ot < 4; ot++) {
+                        v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
+                        for (int kt = 0; kt < 4; kt++) {
+                            v4s w = *(const v4s*)&W[wo + (ot * 16 + lr) * kMlpHid + kt * 16 + 4 * lg];
+                            z = mfma16(w, hin[kt], z);
+                        }
+                        for (int r = 0; r < 4; r++) {
+                            float t = z[r] + P[bo + ot * 16 + 4 * lg + r];
+                            hout[ot][r] = (short)f2bf(t > 0.0f ? t : 0.0f);
+                        }
+                    }
+                    for (int ot = 0; ot < 4; ot++) hin[ot] = hout[ot];
+                }
+                // output layer (16 outputs, 8 used)
+                v4f y = {0.0f, 0.0f, 0.0f, 0.0f};
+                for (int kt = 0; kt < 4; kt++) {
+                    v4s w = *(const v4s*)&W[kBW4 + lr * kMlpHid + kt * 16 + 4 * lg];
+                    y = mfma16(w, hin[kt], y);
+                }
+                // lane l holds outputs 4 lg + r of cell tile * 16 + lr: hand
+                // the 8 used ones to the cell's lane through LDS
+                if (lg < 2)
+                    for (int r = 0; r < 4; r++)
+                        s_my[(tile * 16 + lr) * kMlpUsed + 4 * lg + r] = y[r] + P[kPB4 + 4 * lg + r];
+            }
+            __syncthreads();
+            if (lane < cnt && sres != 0.0f) {
+                const float* yo = &s_my[lane * kMlpUsed];
+                // res = F^T res' sres; J = F^T J' F sgrad
+                const float rp[3] = {yo[0] * sres, yo[1] * sres, yo[2] * sres};
+                const float jxx = yo[3], jyy = yo[4], jxy = yo[5], jxz = yo[6], jyz = yo[7];
+                const float jp[9] = {jxx, jxy, jxz, jxy, jyy, jyz, jxz, jyz, -jxx - jyy};
+                for (int i = 0; i < 3; i++)
+                    lres[i] += f[i] * rp[0] + f[3 + i] * rp[1] + f[6 + i] * rp[2];
+                float t[9];   // J' F
+                for (int i = 0; i < 3; i++)
+                    for (int j = 0; j < 3; j++)
+                        t[3 * i + j] = jp[3 * i] * f[j] + jp[3 * i + 1] * f[3 + j] + jp[3 * i + 2] * f[6 + j];
+                auto jg = [&](int i, int j) {   // (F^T J' F)_ij
+                    return (f[i] * t[j] + f[3 + i] * t[3 + j] + f[6 + i] * t[6 + j]) * sgrad;
+                };
+                lgrad[0] += jg(0, 0); lgrad[1] += jg(1, 1); lgrad[2] += jg(2, 2);
+                lgrad[3] += jg(0, 1); lgrad[4] += jg(0, 2); lgrad[5] += jg(1, 2);
+            }
+            __syncthreads();
+          }
+        };
 
         auto flush = [&](int cnt, bool cells) {
             if (lane < cnt) {
@@ -354,6 +460,9 @@ k_traverse(TravArgs a) {
                     }
                 }
             }
+            if constexpr (LEARNED) {
+                if (cells) learned_flush(cnt);
+            }
             __syncthreads();
         };
 
@@ -372,20 +481,9 @@ k_traverse(TravArgs a) {
             int kind = 0;   // 1 cell, 2 leaf, 3 open
             int4 nd = make_int4(-1, 0, 0, 0);
             if (node >= 0) {
-                double4 c = a.cm[node];
                 nd = a.info[node];
-                if (c.w != 0.0) {
-                    const int level = nd.y >> 8;
-                    const bool leaf = nd.x < 0;
-                    double ddx = c.x < lo0 ? lo0 - c.x : (c.x > hi0 ? c.x - hi0 : 0.0);
-                    double ddy = c.y < lo1 ? lo1 - c.y : (c.y > hi1 ? c.y - hi1 : 0.0);
-                    double ddz = c.z < lo2 ? lo2 - c.z : (c.z > hi2 ? c.z - hi2 : 0.0);
-                    double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                    double hl = ldexp(h, -level);
-                    double size2 = 4.0 * hl * hl;
-                    if (!(shell && leaf) && d2 > 0.0 && size2 < a.theta2 * d2) kind = 1;
-                    else kind = leaf ? 2 : 3;
-                }
+                kind = mac_kind(a.cm[node], nd, lo0, lo1, lo2, hi0, hi1, hi2, h, a.theta2,
+                                shell);
             }
             // accepted cells, in lane order
             uint64_t cm_ = __ballot(kind == 1);
@@ -436,6 +534,19 @@ k_traverse(TravArgs a) {
         if (ncell > 0) flush(ncell, true);
         if (npart > 0) flush(npart, false);
 
+        if constexpr (LEARNED) {
+            // group totals of residual and gradient, then a = res + J (x - centre)
+            for (int o = 32; o > 0; o >>= 1) {
+                for (int r = 0; r < 3; r++) lres[r] += __shfl_xor(lres[r], o, kWave);
+                for (int r = 0; r < 6; r++) lgrad[r] += __shfl_xor(lgrad[r], o, kWave);
+            }
+            if (member) {
+                apx += (double)(lres[0] + lgrad[0] * txf + lgrad[3] * tyf + lgrad[4] * tzf);
+                apy += (double)(lres[1] + lgrad[3] * txf + lgrad[1] * tyf + lgrad[5] * tzf);
+                apz += (double)(lres[2] + lgrad[4] * txf + lgrad[5] * tyf + lgrad[2] * tzf);
+            }
+        }
+
         if (member) {
             const int o = a.order[my];
             if constexpr (PREC == CM_PREC_FP32_SPLIT) {
@@ -463,6 +574,14 @@ k_zero_inactive(int n, const TreeHeader* __restrict__ hdr,
     ax[o] = 0.0; ay[o] = 0.0; az[o] = 0.0;
 }
 
+int form_groups(Engine& e) {
+    int gb = std::min(1024, std::max(1, (e.count + 255) / 256));
+    hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, e.stream, e.hdr, e.nd_info,
+                       e.nd_box, e.cfg.group_size, e.groups);
+    CM_HIP_TRY(e, hipGetLastError());
+    return CM_OK;
+}
+
 int launch_barnes_hut(Engine& e, double softening) {
     const int n = e.count;
     e.stats = cm_force_stats{};
@@ -472,11 +591,16 @@ int launch_barnes_hut(Engine& e, double softening) {
     e.mark(0);
     const uint64_t* keys = nullptr;
     const int32_t* order = nullptr;
+    const bool learned = e.cfg.learned_correction != 0;
+    if (learned && (e.cfg.precision != CM_PREC_FP32_SPLIT || !mlp_ready(e)))
+        return set_error(e, learned && e.cfg.precision != CM_PREC_FP32_SPLIT
+                                ? CM_ERR_INVALID_ARGUMENT : CM_ERR_NOT_INITIALIZED,
+                         "the learned correction needs FP32_SPLIT and a network "
+                         "(cm_mlp_init / cm_mlp_set_params / cm_mlp_train)");
     int rc = build_tree(e, &keys, &order);
     if (rc != CM_OK) return rc;
-    int gb = std::min(1024, std::max(1, (n + 255) / 256));
-    hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, st, e.hdr, e.nd_info,
-                       e.nd_box, e.cfg.group_size, e.groups);
+    rc = form_groups(e);
+    if (rc != CM_OK) return rc;
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[3], st));
     e.mark(1);
     TravArgs a;
@@ -494,17 +618,23 @@ int launch_barnes_hut(Engine& e, double softening) {
     a.levels = e.cfg.morton_levels;
     a.shell = e.cfg.shell_theorem;
     a.counters = &e.hdr->n_cell_int;
+    a.mlp_w = learned ? mlp_weights(e) : nullptr;
+    a.mlp_p = learned ? mlp_params(e) : nullptr;
+    a.quad = e.nd_quad;
     // persistent wavefronts: about 16 per CU, never more than bodies / 8
     int nblocks = std::max(1, std::min(4096, (n + 7) / 8));
     switch (e.cfg.precision) {
         case CM_PREC_FP64:
-            hipLaunchKernelGGL(k_traverse<CM_PREC_FP64>, dim3(nblocks), dim3(kWave), 0, st, a);
+            hipLaunchKernelGGL((k_traverse<CM_PREC_FP64, false>), dim3(nblocks), dim3(kWave), 0, st, a);
             break;
         case CM_PREC_FP64_FAST:
-            hipLaunchKernelGGL(k_traverse<CM_PREC_FP64_FAST>, dim3(nblocks), dim3(kWave), 0, st, a);
+            hipLaunchKernelGGL((k_traverse<CM_PREC_FP64_FAST, false>), dim3(nblocks), dim3(kWave), 0, st, a);
             break;
         default:
-            hipLaunchKernelGGL(k_traverse<CM_PREC_FP32_SPLIT>, dim3(nblocks), dim3(kWave), 0, st, a);
+            if (learned)
+                hipLaunchKernelGGL((k_traverse<CM_PREC_FP32_SPLIT, true>), dim3(nblocks), dim3(kWave), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_traverse<CM_PREC_FP32_SPLIT, false>), dim3(nblocks), dim3(kWave), 0, st, a);
             break;
     }
     hipLaunchKernelGGL(k_zero_inactive, dim3((n + 255) / 256), dim3(256), 0, st, n,

@@ -13,13 +13,14 @@
 //      with 64-lane ballots in original order (stable).
 //   4. k_gather: sorted copies of position / mass / radius (coalesced reads
 //      for the tree and the traversal).
-//   5. k_build_level (one launch per level): level-order octree.  A node with
-//      count > leaf_capacity below the deepest level gets its non-empty
-//      children (binary searches on the sorted keys), allocated after the
-//      whole level by a single-pass scan with decoupled look-back, so children
-//      of a node are contiguous and node numbering is deterministic.
-//   6. k_mass_level (deepest level first): leaf sums over its bodies in sorted
-//      order, internal sums over its children in order; com = sum(m x) / M.
+//   5. k_tree_count / k_tree_scan / k_tree_emit: level-order octree from
+//      local comparisons of neighbouring keys (below).  A node with
+//      count > leaf_capacity below the deepest level has its non-empty
+//      children; children of a node are contiguous, every level is
+//      contiguous, numbering is deterministic.  Leaves sum their bodies in
+//      sorted order while the tile's bodies sit in LDS.
+//   6. k_tree_internal (deepest level first): internal sums over the
+//      children in order; com = sum(m x) / M.
 // Built with -ffp-contract=off: the box, keys and centres of mass are
 // bit-identical to the CPU restatement of the same definitions.
 #include <hip/hip_runtime.h>
@@ -98,7 +99,6 @@ k_bbox_final(int nblocks, const double* __restrict__ part,
         }
         __syncthreads();
     }
-    if (t < kMaxLevels + 2) hdr->tile_counter[t] = 0u;
     if (t == 0) {
         double minx = s[0][0], miny = s[1][0], minz = s[2][0];
         double maxx = s[3][0], maxy = s[4][0], maxz = s[5][0];
@@ -151,21 +151,26 @@ __device__ __forceinline__ uint32_t quantise(double x, double c, double h,
 __global__ void __launch_bounds__(256)
 k_morton(int n, int levels, const TreeHeader* __restrict__ hdr,
          const double* __restrict__ px, const double* __restrict__ py,
-         const double* __restrict__ pz, const uint8_t* __restrict__ act,
-         uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+         const double* __restrict__ pz, const double* __restrict__ m,
+         const uint8_t* __restrict__ act, uint64_t* __restrict__ keys,
+         int32_t* __restrict__ vals, double4* __restrict__ pm4) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     uint64_t sentinel = (levels >= 21) ? ~0ull : ((2ull << (3 * levels)) - 1ull);
     uint64_t k = sentinel;
+    double x = px[i], y = py[i], z = pz[i];
     if (act[i]) {
         double h = hdr->box[3];
-        uint32_t qx = quantise(px[i], hdr->box[0], h, levels);
-        uint32_t qy = quantise(py[i], hdr->box[1], h, levels);
-        uint32_t qz = quantise(pz[i], hdr->box[2], h, levels);
+        uint32_t qx = quantise(x, hdr->box[0], h, levels);
+        uint32_t qy = quantise(y, hdr->box[1], h, levels);
+        uint32_t qz = quantise(z, hdr->box[2], h, levels);
         k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
     }
     keys[i] = k;
     vals[i] = i;
+    // position and mass packed per body, so that the gather after the sort
+    // reads one 32-byte record per body instead of four scattered words
+    pm4[i] = make_double4(x, y, z, m[i]);
 }
 
 // ---------------------------------------------------------- radix sort ----
@@ -279,226 +284,418 @@ k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
 }
 
 __global__ void __launch_bounds__(256)
-k_gather(int n, const int32_t* __restrict__ order, const double* __restrict__ px,
-         const double* __restrict__ py, const double* __restrict__ pz,
-         const double* __restrict__ m, const double* __restrict__ r,
-         double* __restrict__ sorted, int cap) {
+k_gather(int n, const int32_t* __restrict__ order, const double4* __restrict__ pm4,
+         const double* __restrict__ r, double* __restrict__ sorted, int cap) {
     int k = blockIdx.x * 256 + threadIdx.x;
     if (k >= n) return;
     int o = order[k];
-    sorted[k] = px[o];
-    sorted[(size_t)cap + k] = py[o];
-    sorted[2 * (size_t)cap + k] = pz[o];
-    sorted[3 * (size_t)cap + k] = m[o];
+    double4 v = pm4[o];
+    sorted[k] = v.x;
+    sorted[(size_t)cap + k] = v.y;
+    sorted[2 * (size_t)cap + k] = v.z;
+    sorted[3 * (size_t)cap + k] = v.w;
     sorted[4 * (size_t)cap + k] = r[o];
 }
 
 // ---------------------------------------------------------- tree build ----
-__device__ __forceinline__ int lower_digit(const uint64_t* __restrict__ keys,
-                                           int lo, int hi, int shift, uint32_t d) {
-    // first index in [lo, hi) whose digit at `shift` is >= d
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (((uint32_t)(keys[mid] >> shift) & 7u) < d) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+// The level-order octree follows from two local facts of the sorted keys:
+//   d[i] = leading 3-bit digits key[i] shares with key[i - 1] (0 for i = 0):
+//          body i opens a new cell at every level l > d[i];
+//   D[j] = leading digits key[j] shares with key[j + cap] (-1 when j + cap is
+//          not an active body): bodies j .. j + cap all lie in one level-m
+//          cell, which then holds more than cap bodies, iff D[j] >= m.
+// The level-m cell of body i holds more than cap bodies iff one of the
+// windows [j, j + cap] with i - cap <= j <= i lies inside it, i.e. iff
+// max D[i - cap .. i] >= m.  A cell is a node iff its parent cell holds more
+// than cap bodies (a node with more than cap bodies below the deepest level
+// is split into its non-empty children, and counts only shrink going down),
+// so body i opens exactly the nodes of levels d[i] + 1 .. lam[i] with
+// lam[i] = min(levels, 1 + max D[i - cap .. i]); the one of level lam[i] is a
+// leaf.  Nodes are numbered level by level in order of their first body (the
+// digit order of every parent's children), which is a per-level prefix count
+// of opened nodes over the bodies: k_tree_count counts them per tile,
+// k_tree_scan turns the counts into offsets, k_tree_emit writes the nodes and
+// the leaves' sums, k_tree_internal forms internal nodes bottom-up.
+constexpr int kTreeWaves = kTreeTile / kWave;
+constexpr int kTreeHalo = 64;                   // >= largest leaf capacity
+
+__device__ __forceinline__ int common_digits(uint64_t a, uint64_t b, int levels) {
+    uint64_t x = a ^ b;
+    if (x == 0ull) return levels;
+    int p = 63 - __clzll((long long)x);
+    return (3 * levels - 1 - p) / 3;
 }
 
-__device__ __forceinline__ int wave_incl_scan(int v) {
-    int lane = threadIdx.x & (kWave - 1);
-    for (int o = 1; o < kWave; o <<= 1) {
-        int t = __shfl_up(v, o, kWave);
-        if (lane >= o) v += t;
+__global__ void __launch_bounds__(kTreeTile)
+k_tree_count(int levels, int cap, int tstride, const TreeHeader* __restrict__ hdr,
+             const uint64_t* __restrict__ keys, uchar2* __restrict__ dl,
+             uint32_t* __restrict__ tcount) {
+    __shared__ signed char sD[kTreeTile + kTreeHalo];
+    __shared__ uint32_t s_cnt[kMaxLevels + 1][kTreeWaves];
+    const int n = hdr->n_active;
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    const int t0 = blockIdx.x * kTreeTile;
+    // D for j in [t0 - cap, t0 + kTreeTile)
+    for (int q = t; q < kTreeTile + cap; q += kTreeTile) {
+        int j = t0 - cap + q;
+        int D = -1;
+        if (j >= 0 && j + cap < n) D = common_digits(keys[j], keys[j + cap], levels);
+        sD[q] = (signed char)D;
     }
-    return v;
+    __syncthreads();
+    const int i = t0 + t;
+    int d = 0, lam = 0;
+    if (i < n) {
+        d = i == 0 ? 0 : common_digits(keys[i - 1], keys[i], levels);
+        int m = -1;
+        for (int q = t; q <= t + cap; q++) m = max(m, (int)sD[q]);
+        lam = min(levels, m + 1);
+        dl[i] = make_uchar2((unsigned char)d, (unsigned char)lam);
+    }
+    for (int l = 1; l <= levels; l++) {
+        uint64_t mask = __ballot(d < l && l <= lam);
+        if (lane == 0) s_cnt[l][w] = (uint32_t)__popcll(mask);
+    }
+    __syncthreads();
+    if (t >= 1 && t <= levels) {
+        uint32_t s = 0u;
+        for (int q = 0; q < kTreeWaves; q++) s += s_cnt[t][q];
+        tcount[(size_t)t * tstride + blockIdx.x] = s;
+    }
 }
 
-__global__ void __launch_bounds__(kBuildTile)
-k_build_level(int L, int levels, int leaf_cap, int node_cap, int btiles,
-              TreeHeader* __restrict__ hdr, int4* __restrict__ info,
-              const uint64_t* __restrict__ keys,
-              unsigned long long* __restrict__ lookback) {
-    constexpr int kWaves = kBuildTile / kWave;
-    __shared__ int s_tile;
-    __shared__ int s_wsum[kWaves];
-    __shared__ int s_prefix;
-    const int ls = hdr->level_start[L];
-    const int le = hdr->level_start[L + 1];
-    const int size = le - ls;
-    const int ntiles = (size + kBuildTile - 1) / kBuildTile;
-    int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-    if (ntiles == 0) {
-        if (blockIdx.x == 0 && t == 0) {
-            hdr->level_start[L + 2] = le;
-        }
-        return;
-    }
-    unsigned long long* status = lookback + (size_t)L * btiles;
-    for (;;) {
-        if (t == 0) s_tile = (int)atomicAdd(&hdr->tile_counter[L], 1u);
-        __syncthreads();
-        int tile = s_tile;
-        __syncthreads();
-        if (tile >= ntiles) break;
-        int k = ls + tile * kBuildTile + t;
-        bool valid = k < le;
-        int cs[8], cc[8];
-        int nch = 0;
-        int4 me = valid ? info[k] : make_int4(0, 0, 0, 0);
-        if (valid && me.w > leaf_cap && L < levels) {
-            int shift = 3 * (levels - L - 1);
-            int lo = me.z, hi = me.z + me.w;
-            // the 7 boundaries (first index with digit >= d) are searched
-            // together, one step of each per round, so their loads overlap
-            int b[9], blo[8], bhi[8];
-            b[0] = lo; b[8] = hi;
-            for (int d = 1; d < 8; d++) { blo[d] = lo; bhi[d] = hi; }
-            for (int round = 0; round < 32; round++) {
-                bool any = false;
-                for (int d = 1; d < 8; d++) {
-                    if (blo[d] < bhi[d]) {
-                        int mid = (blo[d] + bhi[d]) >> 1;
-                        if (((uint32_t)(keys[mid] >> shift) & 7u) < (uint32_t)d) blo[d] = mid + 1;
-                        else bhi[d] = mid;
-                        any = true;
-                    }
+// One block: per level, exclusive scan of the tile counts (in place), level
+// extents, node total, overflow flag and the root's record.
+__global__ void __launch_bounds__(1024)
+k_tree_scan(int levels, int cap, int ntiles, int tstride, int node_cap,
+            TreeHeader* __restrict__ hdr, uint32_t* __restrict__ tcount,
+            int4* __restrict__ info) {
+    __shared__ uint32_t s_w[kMaxLevels + 1][16];
+    __shared__ uint32_t s_carry[kMaxLevels + 1];
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    if (t <= kMaxLevels) s_carry[t] = 0u;
+    for (int base = 0; base < ntiles; base += 1024) {
+        // every level's count of this thread's tile is loaded at once
+        const int q = base + t;
+        uint32_t v[kMaxLevels + 1];
+#pragma unroll
+        for (int l = 1; l <= kMaxLevels; l++)
+            v[l] = (l <= levels && q < ntiles) ? tcount[(size_t)l * tstride + q] : 0u;
+        uint32_t x[kMaxLevels + 1];
+#pragma unroll
+        for (int l = 1; l <= kMaxLevels; l++) {
+            uint32_t a = v[l];
+            if (l <= levels) {   // uniform
+                for (int o = 1; o < kWave; o <<= 1) {
+                    uint32_t y = __shfl_up(a, o, kWave);
+                    if (lane >= o) a += y;
                 }
-                if (!any) break;
             }
-            for (int d = 1; d < 8; d++) b[d] = blo[d];
-            for (int d = 0; d < 8; d++) {
-                int c = b[d + 1] - b[d];
-                if (c > 0) { cs[nch] = b[d]; cc[nch] = c; nch++; }
-            }
-        }
-        // block exclusive scan of nch
-        int incl = wave_incl_scan(nch);
-        if (lane == kWave - 1) s_wsum[w] = incl;
-        __syncthreads();
-        int woff = 0, agg = 0;
-        for (int q = 0; q < kWaves; q++) {
-            if (q < w) woff += s_wsum[q];
-            agg += s_wsum[q];
-        }
-        int excl = woff + incl - nch;
-        // decoupled look-back over tiles of this level, 64 predecessors per
-        // round (one per lane of the first wavefront)
-        if (w == 0) {
-            unsigned long long prefix = 0ull;
-            if (tile == 0) {
-                if (lane == 0)
-                    __hip_atomic_store(&status[0], (2ull << 32) | (unsigned)agg,
-                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
-                if (lane == 0)
-                    __hip_atomic_store(&status[tile], (1ull << 32) | (unsigned)agg,
-                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-                int p = tile - 1;
-                for (;;) {
-                    int idx = p - lane;
-                    unsigned long long v = idx >= 0
-                        ? __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE,
-                                            __HIP_MEMORY_SCOPE_AGENT)
-                        : (2ull << 32);
-                    unsigned flag = (unsigned)(v >> 32);
-                    if (__ballot(flag == 0u)) continue;     // a predecessor not yet published
-                    uint64_t incl = __ballot(flag == 2u);
-                    int stop = incl ? __ffsll((unsigned long long)incl) - 1 : kWave - 1;
-                    unsigned long long mine = lane <= stop ? (v & 0xffffffffull) : 0ull;
-                    for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o, kWave);
-                    prefix += mine;
-                    if (incl) break;
-                    p -= kWave;
-                }
-                if (lane == 0)
-                    __hip_atomic_store(&status[tile],
-                                       (2ull << 32) | (unsigned)(prefix + (unsigned)agg),
-                                       __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            if (lane == 0) s_prefix = (int)prefix;
+            x[l] = a;
+            if (lane == kWave - 1) s_w[l][w] = a;
         }
         __syncthreads();
-        int first = le + s_prefix + excl;
-        if (valid) {
-            if (nch > 0 && first + nch <= node_cap) {
-                info[k] = make_int4(first, nch | (L << 8), me.z, me.w);
-                for (int c = 0; c < nch; c++)
-                    info[first + c] = make_int4(-1, (L + 1) << 8, cs[c], cc[c]);
-            } else {
-                if (nch > 0) atomicExch(&hdr->overflow, 1);
-                info[k] = make_int4(-1, L << 8, me.z, me.w);
-            }
+#pragma unroll
+        for (int l = 1; l <= kMaxLevels; l++) {
+            if (l > levels) continue;
+            uint32_t woff = 0u;
+            for (int k = 0; k < w; k++) woff += s_w[l][k];
+            if (q < ntiles) tcount[(size_t)l * tstride + q] = s_carry[l] + woff + x[l] - v[l];
         }
-        if (tile == ntiles - 1 && t == 0) {
-            int end = le + s_prefix + agg;
-            if (end > node_cap) end = node_cap;
-            hdr->level_start[L + 2] = end;
-            hdr->n_nodes = end;
-            if (agg > 0) hdr->n_levels_used = L + 2;
+        __syncthreads();
+        if (t >= 1 && t <= levels) {
+            uint32_t tot = 0u;
+            for (int k = 0; k < 16; k++) tot += s_w[t][k];
+            s_carry[t] += tot;
         }
         __syncthreads();
     }
+    long long running = 1;   // the root
+    int used = 1;
+    for (int l = 1; l <= levels; l++) {
+        if (t == 0) hdr->level_start[l] = (int)min(running, (long long)INT32_MAX);
+        uint32_t total = s_carry[l];
+        running += total;
+        if (total > 0u) used = l + 1;
+    }
+    if (t == 0) {
+        int end = (int)min(running, (long long)INT32_MAX);
+        for (int l = levels + 1; l < kMaxLevels + 3; l++) hdr->level_start[l] = end;
+        int over = running > node_cap ? 1 : 0;
+        hdr->overflow = over;
+        hdr->n_nodes = over ? node_cap : end;
+        hdr->n_levels_used = used;
+        int na = hdr->n_active;
+        info[0] = make_int4(na > cap ? 1 : -1, 0, 0, na);
+    }
 }
 
-// Bottom-up mass, centre of mass and body bounding box for the nodes of
-// level L (box: 6 doubles lo x, y, z, hi x, y, z per node).
-__global__ void __launch_bounds__(256)
-k_mass_level(int L, const TreeHeader* __restrict__ hdr,
-             const int4* __restrict__ info, const double* __restrict__ sorted,
-             int cap, double4* __restrict__ cm, double* __restrict__ sums,
-             double* __restrict__ box) {
-    const int ls = hdr->level_start[L];
-    const int le = hdr->level_start[L + 1];
+__device__ __forceinline__ void leaf_accumulate(double x, double y, double z, double mb,
+                                                double& M, double& ax, double& ay,
+                                                double& az, double lo[3], double hi[3]) {
+    M += mb;
+    ax += mb * x;
+    ay += mb * y;
+    az += mb * z;
+    lo[0] = x < lo[0] ? x : lo[0]; lo[1] = y < lo[1] ? y : lo[1]; lo[2] = z < lo[2] ? z : lo[2];
+    hi[0] = x > hi[0] ? x : hi[0]; hi[1] = y > hi[1] ? y : hi[1]; hi[2] = z > hi[2] ? z : hi[2];
+}
+
+__device__ __forceinline__ void store_node_sums(int k, double M, double ax, double ay,
+                                                double az, const double lo[3],
+                                                const double hi[3], double4* cm,
+                                                double* sums, double* box) {
+    double* kb = box + 6 * (size_t)k;
+    kb[0] = lo[0]; kb[1] = lo[1]; kb[2] = lo[2]; kb[3] = hi[0]; kb[4] = hi[1]; kb[5] = hi[2];
+    sums[3 * (size_t)k] = ax;
+    sums[3 * (size_t)k + 1] = ay;
+    sums[3 * (size_t)k + 2] = az;
+    double4 r;
+    if (M > 0.0) { r.x = ax / M; r.y = ay / M; r.z = az / M; }
+    else { r.x = 0.0; r.y = 0.0; r.z = 0.0; }
+    r.w = M;
+    cm[k] = r;
+}
+
+// Writes every node below the root: start, parent, first child (internal
+// nodes; their count, child count and sums come from k_tree_internal), and
+// for leaves the count, mass, sum m x, centre of mass and body box, summed
+// in sorted order from the tile's bodies staged in LDS.
+__global__ void __launch_bounds__(kTreeTile)
+k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict__ hdr,
+            const uchar2* __restrict__ dl, const uint32_t* __restrict__ toff,
+            const double* __restrict__ sorted, int bcap, int4* __restrict__ info,
+            int32_t* __restrict__ parent, double4* __restrict__ cm,
+            double* __restrict__ sums, double* __restrict__ box,
+            float* __restrict__ quad) {
+    constexpr int kS = kTreeTile + kTreeHalo;
+    __shared__ uint64_t s_mask[kMaxLevels + 1][kTreeWaves];
+    __shared__ uint32_t s_woff[kMaxLevels + 1][kTreeWaves];
+    __shared__ int s_ls[kMaxLevels + 3];
+    __shared__ double s_x[kS], s_y[kS], s_z[kS], s_m[kS];
+    __shared__ unsigned char s_d[kS];
+    if (hdr->overflow) return;
+    const int n = hdr->n_active;
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    const int t0 = blockIdx.x * kTreeTile;
     const double* sx = sorted;
-    const double* sy = sorted + cap;
-    const double* sz = sorted + 2 * (size_t)cap;
-    const double* sm = sorted + 3 * (size_t)cap;
-    const double big = 1.7976931348623157e308;
-    for (int k = ls + blockIdx.x * 256 + threadIdx.x; k < le; k += gridDim.x * 256) {
-        int4 nd = info[k];
-        double M = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
-        double lo0 = big, lo1 = big, lo2 = big, hi0 = -big, hi1 = -big, hi2 = -big;
-        if (nd.x < 0) {
-            for (int q = nd.z; q < nd.z + nd.w; q++) {
-                double mb = sm[q];
-                double x = sx[q], y = sy[q], z = sz[q];
-                M += mb;
-                ax += mb * x;
-                ay += mb * y;
-                az += mb * z;
-                lo0 = x < lo0 ? x : lo0; lo1 = y < lo1 ? y : lo1; lo2 = z < lo2 ? z : lo2;
-                hi0 = x > hi0 ? x : hi0; hi1 = y > hi1 ? y : hi1; hi2 = z > hi2 ? z : hi2;
-            }
+    const double* sy = sorted + bcap;
+    const double* sz = sorted + 2 * (size_t)bcap;
+    const double* sm = sorted + 3 * (size_t)bcap;
+    for (int q = t; q < kS; q += kTreeTile) {
+        int j = t0 + q;
+        if (j < n) {
+            s_d[q] = dl[j].x;
+            s_x[q] = sx[j]; s_y[q] = sy[j]; s_z[q] = sz[j]; s_m[q] = sm[j];
         } else {
-            int nch = nd.y & 255;
-            for (int c = nd.x; c < nd.x + nch; c++) {
-                M += cm[c].w;
-                ax += sums[3 * (size_t)c];
-                ay += sums[3 * (size_t)c + 1];
-                az += sums[3 * (size_t)c + 2];
-                const double* cb = box + 6 * (size_t)c;
-                lo0 = cb[0] < lo0 ? cb[0] : lo0; lo1 = cb[1] < lo1 ? cb[1] : lo1;
-                lo2 = cb[2] < lo2 ? cb[2] : lo2; hi0 = cb[3] > hi0 ? cb[3] : hi0;
-                hi1 = cb[4] > hi1 ? cb[4] : hi1; hi2 = cb[5] > hi2 ? cb[5] : hi2;
-            }
+            s_d[q] = 0;
         }
-        double* kb = box + 6 * (size_t)k;
-        kb[0] = lo0; kb[1] = lo1; kb[2] = lo2; kb[3] = hi0; kb[4] = hi1; kb[5] = hi2;
-        sums[3 * (size_t)k] = ax;
-        sums[3 * (size_t)k + 1] = ay;
-        sums[3 * (size_t)k + 2] = az;
-        double4 r;
-        if (M > 0.0) { r.x = ax / M; r.y = ay / M; r.z = az / M; }
-        else { r.x = 0.0; r.y = 0.0; r.z = 0.0; }
-        r.w = M;
-        cm[k] = r;
+    }
+    if (t < kMaxLevels + 3) s_ls[t] = hdr->level_start[t];
+    const int i = t0 + t;
+    int d = 0, lam = 0;
+    if (i < n) {
+        uchar2 v = dl[i];
+        d = v.x;
+        lam = v.y;
+    }
+    for (int l = 1; l <= levels; l++) {
+        uint64_t mask = __ballot(d < l && l <= lam);
+        if (lane == 0) s_mask[l][w] = mask;
+    }
+    __syncthreads();
+    if (t < levels * kTreeWaves) {
+        int l = t / kTreeWaves + 1, ww = t % kTreeWaves;
+        uint32_t s = toff[(size_t)l * tstride + blockIdx.x];
+        for (int q = 0; q < ww; q++) s += (uint32_t)__popcll(s_mask[l][q]);
+        s_woff[l][ww] = s;
+    }
+    __syncthreads();
+    if (i >= n || lam <= d) return;
+    const uint64_t below = (1ull << lane) - 1ull;
+    auto index_at = [&](int l) -> int {
+        return s_ls[l] + (int)s_woff[l][w] + __popcll(s_mask[l][w] & below);
+    };
+    const double big = 1.7976931348623157e308;
+    for (int l = d + 1; l <= lam; l++) {
+        int k = index_at(l);
+        if (k >= node_cap) break;
+        // the level l - 1 node holding body i: opened by i itself when
+        // d < l - 1, else by the last body before i that opened one
+        parent[k] = l == 1 ? 0 : index_at(l - 1) + (d < l - 1 ? 1 : 0) - 1;
+        if (l < lam) {
+            info[k] = make_int4(index_at(l + 1), l << 8, i, 0);
+            continue;
+        }
+        // leaf: bodies i .. end - 1, end = first j > i that opens a level-l
+        // cell (deepest-level leaves may hold more than cap bodies); the
+        // candidates are read 8 at a time so their loads overlap
+        // staged part first (s_d is 0 past the last body, which stops the
+        // search), then, for a deepest-level leaf running past the halo, the
+        // rest from global memory
+        const int lim = t0 + kS;
+        int end = i + 1;
+        for (bool found = false; !found && end < lim;) {
+            int stop = 8;
+#pragma unroll
+            for (int u = 7; u >= 0; u--) {
+                int q = end + u - t0;
+                if (q < kS && (int)s_d[q] < l) stop = u;
+            }
+            found = stop < 8;
+            end = min(end + stop, lim);
+        }
+        if (end == lim) {
+            while (end < n && (int)dl[end].x >= l) end++;
+        }
+        if (end > n) end = n;
+        // sums in sorted order, four bodies loaded per round
+        double M = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+        double lo[3] = {big, big, big}, hi[3] = {-big, -big, -big};
+        const int mid = min(end, lim);
+        for (int j = i; j < mid; j += 4) {
+            double bx[4], by[4], bz[4], bm[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                int q = min(j + u, lim - 1) - t0;
+                bx[u] = s_x[q]; by[u] = s_y[q]; bz[u] = s_z[q]; bm[u] = s_m[q];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (j + u < mid) leaf_accumulate(bx[u], by[u], bz[u], bm[u], M, ax, ay, az, lo, hi);
+        }
+        for (int j = mid; j < end; j++)
+            leaf_accumulate(sx[j], sy[j], sz[j], sm[j], M, ax, ay, az, lo, hi);
+        store_node_sums(k, M, ax, ay, az, lo, hi, cm, sums, box);
+        info[k] = make_int4(-1, l << 8, i, end - i);
+        if (quad) {
+            // second moments about the centre of mass (learned far-field
+            // correction inputs)
+            double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+            if (M > 0.0) { c0 = ax / M; c1 = ay / M; c2 = az / M; }
+            double Q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            for (int j = i; j < end; j++) {
+                int q = j - t0;
+                bool st = j < mid;
+                double dx = (st ? s_x[q] : sx[j]) - c0;
+                double dy = (st ? s_y[q] : sy[j]) - c1;
+                double dz = (st ? s_z[q] : sz[j]) - c2;
+                double mb = st ? s_m[q] : sm[j];
+                Q[0] += mb * dx * dx; Q[1] += mb * dy * dy; Q[2] += mb * dz * dz;
+                Q[3] += mb * dx * dy; Q[4] += mb * dx * dz; Q[5] += mb * dy * dz;
+            }
+#pragma unroll
+            for (int r = 0; r < 6; r++) quad[6 * (size_t)k + r] = (float)Q[r];
+        }
     }
 }
 
-__global__ void k_fill_u64(size_t n, unsigned long long* __restrict__ p) {
-    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
-    for (; i < n; i += (size_t)gridDim.x * 256) p[i] = 0ull;
+// Internal nodes of level L (deepest level first): child count, body count,
+// mass, sum m x, centre of mass = sum m x / M and body box over the children
+// in order; a root that is a leaf sums its bodies.
+__global__ void __launch_bounds__(256)
+k_tree_internal(int L, const TreeHeader* __restrict__ hdr, int4* __restrict__ info,
+                const int32_t* __restrict__ parent, const double* __restrict__ sorted,
+                int bcap, double4* __restrict__ cm, double* __restrict__ sums,
+                double* __restrict__ box, float* __restrict__ quad) {
+    if (hdr->overflow) return;
+    const int ls = hdr->level_start[L];
+    const int le = hdr->level_start[L + 1];
+    const int ce = hdr->level_start[L + 2];
+    const double big = 1.7976931348623157e308;
+    // eight lanes per node, lane u loads child u; the sums are then folded in
+    // child order by every lane of the segment
+    const int u = threadIdx.x & 7;
+    const int seg = (threadIdx.x & (kWave - 1)) & ~7;
+    const int per_block = 256 / 8;
+    for (int kb = ls + blockIdx.x * per_block; kb < le; kb += gridDim.x * per_block) {
+        const int k = kb + (threadIdx.x >> 3);
+        const bool valid = k < le;
+        int4 nd = valid ? info[k] : make_int4(-1, 0, 0, 0);
+        double M = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
+        double lo[3] = {big, big, big}, hi[3] = {-big, -big, -big};
+        if (valid && nd.x < 0 && L == 0 && u == 0) {
+            // a root that is a leaf (at most cap bodies)
+            for (int j = 0; j < nd.w; j++)
+                leaf_accumulate(sorted[j], sorted[bcap + j], sorted[2 * (size_t)bcap + j],
+                                sorted[3 * (size_t)bcap + j], M, ax, ay, az, lo, hi);
+            store_node_sums(k, M, ax, ay, az, lo, hi, cm, sums, box);
+            if (quad) {
+                double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+                if (M > 0.0) { c0 = ax / M; c1 = ay / M; c2 = az / M; }
+                double Q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+                for (int j = 0; j < nd.w; j++) {
+                    double mb = sorted[3 * (size_t)bcap + j];
+                    double dx = sorted[j] - c0, dy = sorted[bcap + j] - c1;
+                    double dz = sorted[2 * (size_t)bcap + j] - c2;
+                    Q[0] += mb * dx * dx; Q[1] += mb * dy * dy; Q[2] += mb * dz * dz;
+                    Q[3] += mb * dx * dy; Q[4] += mb * dx * dz; Q[5] += mb * dy * dz;
+                }
+                for (int r = 0; r < 6; r++) quad[6 * (size_t)k + r] = (float)Q[r];
+            }
+            M = 0.0; ax = 0.0; ay = 0.0; az = 0.0;
+        }
+        const int c = nd.x + u;
+        const bool mine = valid && nd.x >= 0 && c < ce && parent[c] == k;
+        double cmw = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        double b[6] = {big, big, big, -big, -big, -big};
+        int cnt = 0;
+        if (mine) {
+            cmw = cm[c].w;
+            s0 = sums[3 * (size_t)c];
+            s1 = sums[3 * (size_t)c + 1];
+            s2 = sums[3 * (size_t)c + 2];
+            cnt = info[c].w;
+            const double* cb = box + 6 * (size_t)c;
+#pragma unroll
+            for (int q = 0; q < 6; q++) b[q] = cb[q];
+        }
+        const uint64_t own = __ballot(mine);
+        const int nch = __popcll((own >> seg) & 0xffull);
+        int count = 0;
+        for (int q = 0; q < 8; q++) {
+            int src = seg + q;
+            double vm = __shfl(cmw, src, kWave);
+            double v0 = __shfl(s0, src, kWave);
+            double v1 = __shfl(s1, src, kWave);
+            double v2 = __shfl(s2, src, kWave);
+            int vc = __shfl(cnt, src, kWave);
+            double vb[6];
+#pragma unroll
+            for (int r = 0; r < 6; r++) vb[r] = __shfl(b[r], src, kWave);
+            if (q < nch) {
+                M += vm; ax += v0; ay += v1; az += v2; count += vc;
+                lo[0] = vb[0] < lo[0] ? vb[0] : lo[0]; lo[1] = vb[1] < lo[1] ? vb[1] : lo[1];
+                lo[2] = vb[2] < lo[2] ? vb[2] : lo[2]; hi[0] = vb[3] > hi[0] ? vb[3] : hi[0];
+                hi[1] = vb[4] > hi[1] ? vb[4] : hi[1]; hi[2] = vb[5] > hi[2] ? vb[5] : hi[2];
+            }
+        }
+        if (quad) {
+            // parallel axis: Q = sum over children of Q_c + m_c (c_c - c)(c_c - c)^T
+            double c0 = 0.0, c1 = 0.0, c2 = 0.0;
+            if (M > 0.0) { c0 = ax / M; c1 = ay / M; c2 = az / M; }
+            double Q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (mine && cmw > 0.0) {
+                double dx = s0 / cmw - c0, dy = s1 / cmw - c1, dz = s2 / cmw - c2;
+                const float* qc = quad + 6 * (size_t)c;
+                Q[0] = (double)qc[0] + cmw * dx * dx; Q[1] = (double)qc[1] + cmw * dy * dy;
+                Q[2] = (double)qc[2] + cmw * dz * dz; Q[3] = (double)qc[3] + cmw * dx * dy;
+                Q[4] = (double)qc[4] + cmw * dx * dz; Q[5] = (double)qc[5] + cmw * dy * dz;
+            }
+#pragma unroll
+            for (int r = 0; r < 6; r++) {
+                Q[r] += __shfl_xor(Q[r], 1, kWave);
+                Q[r] += __shfl_xor(Q[r], 2, kWave);
+                Q[r] += __shfl_xor(Q[r], 4, kWave);
+            }
+            if (valid && nd.x >= 0 && u == 0)
+                for (int r = 0; r < 6; r++) quad[6 * (size_t)k + r] = (float)Q[r];
+        }
+        if (valid && nd.x >= 0 && u == 0) {
+            store_node_sums(k, M, ax, ay, az, lo, hi, cm, sums, box);
+            info[k] = make_int4(nd.x, nch | (L << 8), nd.z, count);
+        }
+    }
 }
 
 // Stable LSD radix sort of n (key, value) pairs over 8-bit digits
@@ -537,30 +734,34 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
                        e.bbox_cnt, e.hdr, e.nd_info);
     int gb = (n + 255) / 256;
     hipLaunchKernelGGL(k_morton, dim3(gb), dim3(256), 0, st, n, levels, e.hdr,
-                       e.f[PX], e.f[PY], e.f[PZ], e.act, e.keys_a, e.vals_a);
+                       e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, e.keys_a, e.vals_a,
+                       e.pm4);
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
     int passes = (3 * levels + 1 + 7) / 8;
     uint64_t* ka = e.keys_a; uint64_t* kb = e.keys_b;
     int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
     radix_sort_pairs(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist);
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
-    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.f[PX],
-                       e.f[PY], e.f[PZ], e.f[MASS], e.f[RAD], e.sorted, e.capacity);
-    int btiles = (e.node_cap + kBuildTile - 1) / kBuildTile;
-    size_t nlb = (size_t)(levels + 1) * btiles;
-    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, st, nlb, e.lookback);
-    for (int L = 0; L <= levels; L++) {
+    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4, e.f[RAD],
+                       e.sorted, e.capacity);
+    const int cap = e.cfg.leaf_capacity;
+    const int ntiles = std::max(1, (n + kTreeTile - 1) / kTreeTile);
+    const int tstride = tree_tiles(e.capacity);
+    hipLaunchKernelGGL(k_tree_count, dim3(ntiles), dim3(kTreeTile), 0, st, levels, cap,
+                       tstride, e.hdr, ka, e.tree_dl, e.tree_tcount);
+    hipLaunchKernelGGL(k_tree_scan, dim3(1), dim3(1024), 0, st, levels, cap, ntiles,
+                       tstride, e.node_cap, e.hdr, e.tree_tcount, e.nd_info);
+    hipLaunchKernelGGL(k_tree_emit, dim3(ntiles), dim3(kTreeTile), 0, st, levels,
+                       e.node_cap, tstride, e.hdr, e.tree_dl, e.tree_tcount, e.sorted,
+                       e.capacity, e.nd_info, e.nd_parent, e.nd_cm, e.nd_sum, e.nd_box,
+                       e.cfg.learned_correction ? e.nd_quad : nullptr);
+    // deepest internal nodes live at level levels - 1
+    for (int L = levels - 1; L >= 0; L--) {
         double maxsz = std::min((double)n, L < 20 ? std::ldexp(1.0, 3 * L) : 1e18);
-        int g = (int)std::min(2048.0, std::max(1.0, std::ceil(maxsz / kBuildTile)));
-        hipLaunchKernelGGL(k_build_level, dim3(g), dim3(kBuildTile), 0, st, L,
-                           levels, e.cfg.leaf_capacity, e.node_cap, btiles, e.hdr,
-                           e.nd_info, ka, e.lookback);
-    }
-    for (int L = levels; L >= 0; L--) {
-        double maxsz = std::min((double)n, L < 20 ? std::ldexp(1.0, 3 * L) : 1e18);
-        int g = (int)std::min(1024.0, std::max(1.0, std::ceil(maxsz / 256)));
-        hipLaunchKernelGGL(k_mass_level, dim3(g), dim3(256), 0, st, L, e.hdr,
-                           e.nd_info, e.sorted, e.capacity, e.nd_cm, e.nd_sum, e.nd_box);
+        int g = (int)std::min(2048.0, std::max(1.0, std::ceil(maxsz / 32)));
+        hipLaunchKernelGGL(k_tree_internal, dim3(g), dim3(256), 0, st, L, e.hdr,
+                           e.nd_info, e.nd_parent, e.sorted, e.capacity, e.nd_cm,
+                           e.nd_sum, e.nd_box, e.cfg.learned_correction ? e.nd_quad : nullptr);
     }
     CM_HIP_TRY(e, hipGetLastError());
     *keys_out = ka;

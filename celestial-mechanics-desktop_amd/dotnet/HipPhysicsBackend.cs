@@ -26,6 +26,7 @@ internal static unsafe class CmHip
         public double Theta;
         public int LeafCapacity;
         public int GroupSize;
+        public int LearnedCorrection; // 1: MLP far-field correction (precision 2 only)
     }
 
     [StructLayout(LayoutKind.Sequential)]

@@ -5,7 +5,8 @@
  * surface (mfscpayload-690/celestial-mechanics-desktop,
  * src/CelestialMechanics.Physics).  A C# class implementing
  * IPhysicsComputeBackend / IGravityModelAwareBackend binds these entry points
- * with [DllImport("cmhip")]; see INTEGRATION.md for that binding.
+ * with [DllImport("cmhip")] (celestial-mechanics-desktop_amd/dotnet/
+ * HipPhysicsBackend.cs).
  *
  * Conventions (modelled on the reference's native ABI,
  * engine/include/celestial/interop/native_api.h:30-365, and its ErrorCode,
@@ -69,6 +70,8 @@ typedef struct {
     double theta;              /* BarnesHutBackend.Theta, clamped to >= 0.2 */
     int32_t leaf_capacity;     /* bodies per leaf bucket, 1..64 */
     int32_t group_size;        /* max bodies per target group (one wavefront), 1..64 */
+    int32_t learned_correction;/* 1: add the MLP far-field correction (Barnes-Hut,
+                                  FP32_SPLIT only; no reference counterpart) */
 } cm_backend_config;
 
 /* Pointers to the 18 BodySoA arrays (SoA/BodySoA.cs:42-69).  On upload a
@@ -221,6 +224,51 @@ int32_t cm_ensemble_step(void* ensemble, double softening, double dt,
 int32_t cm_ensemble_energy(void* ensemble, double softening, double* energy);
 int32_t cm_ensemble_get_stats(void* ensemble, cm_ensemble_stats* out);
 
+/* ---- learned far-field correction (no reference counterpart: the
+ * reference's Barnes-Hut is monopole-only, BarnesHutBackend.cs:392-459) ---- */
+/* A 16 -> 64 -> 64 -> 64 -> 16 ReLU MLP (bf16 weights and activations on
+ * MFMA, fp32 accumulation) evaluated per (target group, accepted cell) pair of
+ * the traversal; it predicts the cell's monopole truncation residual at the
+ * group centre and its gradient, and the traversal adds
+ * sum over cells (residual + gradient . (x - centre)) to each body when
+ * cm_backend_config.learned_correction = 1 (FP32_SPLIT only).
+ * Parameters: CM_MLP_PARAMS floats, per layer W[out][in] row-major then b[out]
+ * (layers 64x16, 64x64, 64x64, 16x64).  Inputs per pair: r/|r| (3), h/|r|,
+ * Q/(M h^2) (xx, yy, zz, xy, xz, yz), 6 zeros, with r = cell centre of mass
+ * minus group centre, h the cell half size, Q the cell's second moments about
+ * its centre of mass; outputs 0..2 residual / (M h^2 / |r|^4), 3..7 gradient
+ * (xx, yy, xy, xz, yz) / (M h^2 / |r|^5), 8..15 unused. */
+#define CM_MLP_IN 16
+#define CM_MLP_OUT 16
+#define CM_MLP_USED 8
+#define CM_MLP_PARAMS 10448
+/* He-uniform weights (SplitMix64 from seed), zero biases; resets Adam. */
+int32_t cm_mlp_init(void* engine, uint64_t seed);
+int32_t cm_mlp_set_params(void* engine, const float* params, int32_t n);
+int32_t cm_mlp_get_params(void* engine, float* params, int32_t n);
+/* outputs y[n][16] for inputs x[n][16] */
+int32_t cm_mlp_forward(void* engine, const float* x, int32_t n, float* y);
+/* gradient (CM_MLP_PARAMS floats) and value of the mean squared error over
+ * the 8 used outputs of one batch x[n][16], targets t[n][8] */
+int32_t cm_mlp_gradient(void* engine, const float* x, const float* t, int32_t n,
+                        float* grad, float* loss);
+/* Training pairs from the current state: builds the tree, samples up to
+ * max_groups target groups (seeded), records each accepted cell of >= 2
+ * bodies (at most max_pairs) with its inputs and its FP64 direct-sum targets
+ * (softened by `softening`, like the traversal).  Kept on the device. */
+int32_t cm_mlp_collect(void* engine, double softening, int32_t max_groups, uint64_t seed,
+                       int32_t max_pairs, int32_t* n_pairs);
+/* copies the first `cap` collected pairs (each array may be NULL): inputs
+ * x[.][16], targets t[.][8], pair_info[.][4] = (sample rank, node, first body,
+ * end body of the target group in sorted order) and the group centre [.][3];
+ * pairs are ordered by sampled group (in first-body order), then node */
+int32_t cm_mlp_get_pairs(void* engine, float* x, float* t, int32_t* pair_info,
+                         double* centre, int32_t cap);
+/* Adam (lr, beta 0.9 / 0.999) over the collected pairs, shuffled per epoch
+ * (seeded); loss_per_epoch (may be NULL) receives the mean batch loss. */
+int32_t cm_mlp_train(void* engine, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                     float* loss_per_epoch);
+
 /* ---- synthetic scenarios (host-side, deterministic from a seed) ----------- */
 /* Planetary systems for the ensemble: system sizes uniform in [n_min, n_max];
  * each is a 1 Msun star plus planets with log-uniform masses in
@@ -232,10 +280,6 @@ int32_t cm_generate_ensemble(int32_t n_systems, uint64_t seed, int32_t n_min,
                              int32_t n_max, double m_min, double m_max,
                              double a_min, double a_max, double sigma_e,
                              int32_t* offsets, const cm_soa_ptrs* out);
-/* PlummerSphere.Generate (ScenarioGenerators/PlummerSphere.cs:41-116) with a
- * SplitMix64 stream; mass_fraction_cut < 1 truncates the sphere at the radius
- * enclosing that mass fraction; com_frame subtracts the centre-of-mass
- * position and velocity.  Writes pos/vel/mass/radius of n bodies. */
 /* Cold thin disk around a 1 Msun star (no reference generator exists for it;
  * it stands in for the reference's body setup of collision scenarios):
  * body 0 is the star (radius star_radius, density consistent with it),
@@ -249,6 +293,10 @@ int32_t cm_generate_disk(int32_t n, uint64_t seed, double star_radius,
                          double a_min, double a_max, double sigma_e,
                          double sigma_i, double m_min, double m_max,
                          double density, const cm_soa_ptrs* out);
+/* PlummerSphere.Generate (ScenarioGenerators/PlummerSphere.cs:41-116) with a
+ * SplitMix64 stream; mass_fraction_cut < 1 truncates the sphere at the radius
+ * enclosing that mass fraction; com_frame subtracts the centre-of-mass
+ * position and velocity.  Writes pos/vel/mass/radius of n bodies. */
 int32_t cm_generate_plummer(int32_t n, double total_mass, double scale_length,
                             uint64_t seed, double mass_fraction_cut,
                             int32_t com_frame, const cm_soa_ptrs* out);
