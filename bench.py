@@ -5,19 +5,33 @@ Plummer sphere (total mass 1, scale radius 1 AU, SplitMix64 seed 42, truncated
 at the 0.999 mass fraction, centre-of-mass frame), softening 1e-4, 30-bit
 Morton keys, Verlet dt = 1e-3.  A step is one full SoAVerletIntegrator.Step
 on the device: kick-drift, tree rebuild (bbox, keys, sort, octree, mass) and
-force traversal, kick-rotate.
+force traversal, kick-rotate.  Inputs are resident in HBM before timing.
 
-The fast path (FP32 split precision) and the plain FP64 GPU path (same sort,
-tree and traversal, FP64 arithmetic) are each timed over K steps after W
-warm-up steps, bracketed by device synchronisation.  Force accuracy is the
-median per-body relative error against an FP64 direct sum on a seeded
-4096-body sample, evaluated on the final state.
+Timed lines, each over K steps after W warm-up steps, bracketed by device
+synchronisation:
+  * value: FP32 split precision at theta = 0.5;
+  * fp64_baseline: the plain FP64 GPU path (same sort, tree and traversal,
+    FP64 arithmetic, theta = 0.5, no learned term) -- the ratio's denominator;
+  * theta_tuned: FP32 split at a wider theta (default 0.65);
+  * learned: FP32 split with the learned far-field correction at
+    --theta-learned, the network trained by the library's own kernels on
+    FP64 direct-sum residuals of a seeded subsample of about 65,536 target
+    bodies (whole target groups of the same state's tree).
+Force accuracy: median per-body relative error against an FP64 direct sum on
+a seeded 4096-body sample, evaluated on the final state.
+
+--workload merger: the learned-correction configuration instead (two Plummer
+spheres of 2^21 bodies each, mass 0.5, a = 1 AU, centres 10 AU apart,
+approaching at 0.3 x the escape speed), theta in {0.5, 0.7, 0.9} with and
+without the network.
 
 Prints one JSON line (value = fast-path body-steps/s).
 """
 import argparse
+import ctypes
 import json
 import os
+import re
 import sys
 import time
 
@@ -27,18 +41,26 @@ sys.path.insert(0, os.path.join(ROOT, "celestial-mechanics-desktop_amd"))
 import numpy as np  # noqa: E402
 
 PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
-PEAK_FP64_VALU_TFLOPS = 78.6
 FLOPS_PER_INTERACTION = 20
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_traverse_plummer1M_traffic.md")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample):
-    n = soa.count
+def make_backend(cmhip, n, precision, theta, params=None):
     be = cmhip.HipPhysicsBackend(n, precision=precision, theta=theta, morton_levels=10,
                                  leaf_capacity=16, group_size=64)
+    if params is not None:
+        be.mlp_set_params(params)
+        be.configure(learned_correction=1)
+    return be
+
+
+def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample, params=None):
+    n = soa.count
+    be = make_backend(cmhip, n, precision, theta, params)
     be.upload(soa)
     be.step(eps, dt, warmup)
     be.reset_stats()
@@ -71,6 +93,46 @@ def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample):
     }
 
 
+def host_roundtrip(cmhip, soa, theta, eps, reps=5):
+    """IPhysicsComputeBackend.ComputeForces as the C# host calls it every step:
+    upload the host BodySoA, evaluate forces (FP32 split), download AccX/Y/Z.
+    PCIe-inclusive; reported beside the device-resident value, never as it."""
+    n = soa.count
+    be = make_backend(cmhip, n, cmhip.PREC_FP32_SPLIT, theta)
+    work = soa.copy()
+    be.compute_forces(work, eps)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        be.compute_forces(work, eps)
+    wall = (time.perf_counter() - t0) / reps
+    be.close()
+    return {"ms_per_force_eval": 1e3 * wall, "body_evals_per_s": n / wall}
+
+
+def train_network(cmhip, soa, theta, eps, n_sub=65536, seed=2024, epochs=20):
+    """The learned correction's network, trained by the library's kernels on
+    direct-sum residuals for a seeded subsample of about n_sub target bodies:
+    whole target groups of the full-resolution tree of `soa`, drawn by seed, so
+    the cells seen in training are the cells the traversal accepts."""
+    n = soa.count
+    be = make_backend(cmhip, n, cmhip.PREC_FP32_SPLIT, theta)
+    be.upload(soa)
+    be.compute_forces_resident(eps)
+    n_groups = be.stats()["n_groups"]
+    max_groups = max(1, int(round(n_sub * n_groups / n)))
+    be.mlp_init(seed)
+    t0 = time.perf_counter()
+    npairs = be.mlp_collect(eps, max_groups=max_groups, seed=seed, max_pairs=1 << 23)
+    losses = be.mlp_train(epochs=epochs, batch=1024, lr=2e-3, seed=seed)
+    secs = time.perf_counter() - t0
+    params = be.mlp_params()
+    be.close()
+    return params, {"pairs": npairs, "epochs": epochs, "loss_first": float(losses[0]),
+                    "loss_last": float(losses[-1]), "seconds": secs,
+                    "target_groups": max_groups,
+                    "target_bodies_approx": int(round(max_groups * n / n_groups))}
+
+
 def cpu_baseline(soa, eps, dt, threads):
     """Reference CPU path restated in C (oracle): BarnesHutBackend with serial
     insertion + OpenMP traversal inside SoAVerletIntegrator.Step, one step."""
@@ -92,33 +154,122 @@ def cpu_baseline(soa, eps, dt, threads):
             "seconds": t}
 
 
+def traffic_from_profile():
+    """HBM bytes per k_traverse<FP32_SPLIT> launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", fh.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def copy_calibration(nbytes=1 << 28, reps=5):
+    """Device-to-device copies of a known size (the PMC scale reference)."""
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_bytes": nbytes, "copies": reps}), flush=True)
+
+
+def merger(args, cmhip):
+    eps, dt = 1e-4, 1e-3
+    n = args.n if args.n_set else 1 << 22
+    soa = cmhip.generate_merger(n, seed=42)
+    sample = np.sort(np.random.default_rng(4096).choice(n, 4096, replace=False)).astype(np.int32)
+    res = {}
+    for theta in (0.5, 0.7, 0.9):
+        log(f"merger theta={theta} plain ...")
+        plain = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup,
+                         eps, dt, sample)
+        log(f"merger theta={theta}: training on a 65,536-body subsample ...")
+        params, tr = train_network(cmhip, soa, theta, eps)
+        learned = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup,
+                           eps, dt, sample, params=params)
+        res[str(theta)] = {
+            "plain": {k: plain[k] for k in ("body_steps_per_s", "ms_per_step",
+                                            "force_err_median", "interactions_per_body")},
+            "learned": {k: learned[k] for k in ("body_steps_per_s", "ms_per_step",
+                                                "force_err_median", "interactions_per_body")},
+            "training": tr}
+        log(json.dumps(res[str(theta)]))
+    log("plain FP64 GPU path (theta=0.5) ...")
+    base = run_path(cmhip, soa, cmhip.PREC_FP64_FAST, 0.5, args.steps, args.warmup, eps, dt,
+                    sample)
+    line = {"metric": "BH body-steps/s, learned far-field correction (two-Plummer merger)",
+            "value": res["0.5"]["plain"]["body_steps_per_s"], "unit": "body-steps/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": res["0.5"]["plain"]["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32-split (f64 state)",
+            "data": "synthetic",
+            "config": {"workload": "two_plummer_merger_bh_verlet", "n_bodies": n,
+                       "softening": eps, "dt": dt, "separation_au": 10.0,
+                       "speed_fraction_of_escape": 0.3, "seed": 42},
+            "fp64_baseline": {"value": base["body_steps_per_s"],
+                              "force_err_median": base["force_err_median"]},
+            "by_theta": res}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--theta", type=float, default=0.5)
-    ap.add_argument("--theta-tuned", type=float, default=None,
-                    help="also time the fast path at this theta")
+    ap.add_argument("--theta-tuned", type=float, default=0.65,
+                    help="also time the fast path at this theta (0 to skip)")
+    ap.add_argument("--theta-learned", type=float, default=0.75,
+                    help="also time the learned correction at this theta (0 to skip)")
+    ap.add_argument("--workload", choices=("plummer", "merger"), default="plummer")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--fast-only", action="store_true",
+                    help="time only the FP32 theta = 0.5 line (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run device copies of known size (PMC scale reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
-
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    args.n_set = args.n is not None
     import cmhip
+    if args.workload == "merger":
+        merger(args, cmhip)
+        return
+    if args.fast_only:
+        args.no_fp64 = True
+        args.no_cpu_baseline = True
+        args.theta_tuned = 0.0
+        args.theta_learned = 0.0
+    n = args.n or (1 << 20)
     eps, dt = 1e-4, 1e-3
-    soa = cmhip.generate_plummer(args.n, 1.0, 1.0, seed=42, mass_fraction_cut=0.999,
+    soa = cmhip.generate_plummer(n, 1.0, 1.0, seed=42, mass_fraction_cut=0.999,
                                  com_frame=True)
-    sample = np.sort(np.random.default_rng(4096).choice(args.n, min(4096, args.n),
+    sample = np.sort(np.random.default_rng(4096).choice(n, min(4096, n),
                                                          replace=False)).astype(np.int32)
     log(f"fast path (FP32 split, theta={args.theta}) ...")
     fast = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta, args.steps, args.warmup,
                     eps, dt, sample)
     log(json.dumps(fast))
+    roundtrip = None
+    if not args.fast_only:
+        log("host round trip (upload, forces, download) ...")
+        roundtrip = host_roundtrip(cmhip, soa, args.theta, eps)
+        log(json.dumps(roundtrip))
     base = None
     if not args.no_fp64:
         log("plain FP64 GPU path (theta=0.5) ...")
@@ -131,6 +282,21 @@ def main():
         tuned = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta_tuned, args.steps,
                          args.warmup, eps, dt, sample)
         log(json.dumps(tuned))
+    learned = None
+    if args.theta_learned:
+        log(f"learned correction at theta={args.theta_learned}: training ...")
+        params, tr = train_network(cmhip, soa, args.theta_learned, eps)
+        log(json.dumps(tr))
+        learned = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta_learned, args.steps,
+                           args.warmup, eps, dt, sample, params=params)
+        learned["training"] = tr
+        log(json.dumps(learned))
+        log(f"plain FP32 split at theta={args.theta_learned} (the method without the network) ...")
+        same = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta_learned, args.steps,
+                        args.warmup, eps, dt, sample)
+        learned["without_network"] = {"value": same["body_steps_per_s"],
+                                      "force_err_median": same["force_err_median"],
+                                      "ms_per_step": same["ms_per_step"]}
 
     achieved = (FLOPS_PER_INTERACTION * fast["acc_interactions"]
                 / (fast["acc_ms_force"] * 1e-3) / 1e12)
@@ -147,7 +313,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32-split (f64 state)",
         "data": "synthetic",
-        "config": {"workload": "plummer_bh_verlet", "n_bodies": args.n, "theta": args.theta,
+        "config": {"workload": "plummer_bh_verlet", "n_bodies": n, "theta": args.theta,
                    "softening": eps, "dt": dt, "morton_bits": 30, "leaf_capacity": 16,
                    "plummer_mass": 1.0, "plummer_a_au": 1.0, "mass_fraction_cut": 0.999,
                    "seed": 42},
@@ -156,9 +322,13 @@ def main():
         "interactions_per_body": fast["interactions_per_body"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
-                     "traffic": None, "kernel": "k_traverse<FP32_SPLIT>",
-                     "ms_kernel_avg": fast["ms_force_kernel_avg"]},
+                     "traffic": traffic_from_profile(), "kernel": "k_traverse<FP32_SPLIT>",
+                     "ms_kernel_avg": fast["ms_force_kernel_avg"],
+                     "flops_per_launch": (FLOPS_PER_INTERACTION * fast["acc_interactions"]
+                                          / max(1, fast["evaluations"]))},
     }
+    if roundtrip:
+        line["host_roundtrip_pcie"] = roundtrip
     if base:
         line["fp64_baseline"] = {"value": base["body_steps_per_s"],
                                  "ms_per_step": base["ms_per_step"],
@@ -166,13 +336,20 @@ def main():
                                  "interactions_per_body": base["interactions_per_body"],
                                  "ms_force_kernel_avg": base["ms_force_kernel_avg"]}
         line["speedup_vs_fp64_gpu"] = fast["body_steps_per_s"] / base["body_steps_per_s"]
-    if tuned:
-        line["theta_tuned"] = {"theta": args.theta_tuned, "value": tuned["body_steps_per_s"],
-                               "force_err_median": tuned["force_err_median"],
-                               "ms_per_step": tuned["ms_per_step"]}
+    for key, rec, th in (("theta_tuned", tuned, args.theta_tuned),
+                         ("learned", learned, args.theta_learned)):
+        if not rec:
+            continue
+        line[key] = {"theta": th, "value": rec["body_steps_per_s"],
+                     "force_err_median": rec["force_err_median"],
+                     "ms_per_step": rec["ms_per_step"],
+                     "interactions_per_body": rec["interactions_per_body"]}
         if base:
-            line["theta_tuned"]["speedup_vs_fp64_gpu"] = (tuned["body_steps_per_s"]
-                                                          / base["body_steps_per_s"])
+            line[key]["speedup_vs_fp64_gpu"] = rec["body_steps_per_s"] / base["body_steps_per_s"]
+        if "training" in rec:
+            line[key]["training"] = rec["training"]
+        if "without_network" in rec:
+            line[key]["without_network"] = rec["without_network"]
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads ...")

@@ -41,8 +41,12 @@ struct TreeHeader {
     unsigned long long n_body_int;
     unsigned long long acc_int;   // cell + body terms since the last stats reset
     int32_t n_groups;             // target groups formed for the traversal
-    uint32_t work_next;           // next group index handed to a wavefront
+    uint32_t work_next[8];        // per-XCD queues: next group of each eighth
 };
+
+// Workgroups of one launch are dispatched round-robin over the 8 XCDs of an
+// MI355X (each with its own L2): blockIdx.x % kXcds is the block's XCD.
+constexpr int kXcds = 8;
 
 constexpr int kWindow = 1024;     // force evaluations timed per host sync
 
@@ -81,7 +85,9 @@ struct Engine {
     uchar2* tree_dl = nullptr;         // per sorted body: shared digits with the
                                        // previous key, deepest level it opens
     uint32_t* tree_tcount = nullptr;   // (kMaxLevels + 1) x tree_tiles: nodes per tile
-    int2* groups = nullptr;            // target groups: sorted body range [x, y)
+    int2* groups = nullptr;            // target groups: sorted body range [x, y), in body order
+    int32_t* gmark = nullptr;          // per sorted body: end of the group starting there, or 0
+    int32_t* gtile = nullptr;          // group counts / offsets per tile of 1024 bodies
     float* nd_quad = nullptr;          // 6 x node_cap second moments (learned correction)
 
     // learned far-field correction (k_mlp.hip)

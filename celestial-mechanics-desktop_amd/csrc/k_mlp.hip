@@ -46,6 +46,7 @@ struct Mlp {
     int32_t* idx = nullptr;       // batch order, pair_cap
     int32_t* gsel = nullptr;      // sampled groups
     int32_t* counter = nullptr;   // pairs found
+    double* loss_acc = nullptr;   // summed batch losses of the running epoch
     int pair_cap = 0;
     int gsel_cap = 0;
     int n_pairs = 0;
@@ -213,8 +214,15 @@ k_mlp_train(const uint16_t* __restrict__ wb, const uint16_t* __restrict__ wtb,
 __global__ void __launch_bounds__(256)
 k_mlp_adam(const float* __restrict__ grad_part, int nparts, float* __restrict__ params,
            float* __restrict__ m, float* __restrict__ v, float lr, float bc1, float bc2,
-           int update, float* __restrict__ grad_sum) {
+           int update, float* __restrict__ grad_sum, const float* __restrict__ loss_part,
+           float loss_scale, double* __restrict__ loss_acc) {
     int p = blockIdx.x * 256 + threadIdx.x;
+    if (p == 0 && loss_acc) {
+        // running sum of batch losses (mean squared error of each batch)
+        double l = 0.0;
+        for (int w = 0; w < nparts; w++) l += (double)loss_part[w];
+        *loss_acc += l * (double)loss_scale;
+    }
     if (p >= kMlpParams) return;
     float g = 0.0f;
     for (int w = 0; w < nparts; w++) g += grad_part[(size_t)w * kMlpParams + p];
@@ -400,6 +408,7 @@ static int ensure_mlp(Engine& e) {
     alloc((void**)&m->wtb, kMlpWeights * 2);
     alloc((void**)&m->grad_sum, kMlpParams * 4);
     alloc((void**)&m->counter, 4);
+    alloc((void**)&m->loss_acc, sizeof(double));
     if (!ok) return set_error(e, CM_ERR_OUT_OF_MEMORY, "mlp buffers");
     return CM_OK;
 }
@@ -450,7 +459,7 @@ void mlp_free(Engine& e) {
     if (!m) return;
     void* ptrs[] = {m->params, m->adam_m, m->adam_v, m->wb, m->wtb, m->grad_part,
                     m->grad_sum, m->loss_part, m->pairs, m->pair_c, m->px, m->pt,
-                    m->idx, m->gsel, m->counter};
+                    m->idx, m->gsel, m->counter, m->loss_acc};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     delete m;
     e.mlp = nullptr;
@@ -486,9 +495,11 @@ static int upload_params(Engine& e, const float* host) {
     return CM_OK;
 }
 
-// one gradient (and optionally Adam step) over samples idx[0 .. n)
+// one gradient (and optionally Adam step) over samples idx[0 .. n); the batch
+// loss is returned in *loss (a host sync) and/or added to the device sum
+// *loss_acc (no sync)
 static int train_batch(Engine& e, const float* x, const float* t, const int32_t* idx,
-                       int n, float lr, int update, float* loss) {
+                       int n, float lr, int update, float* loss, double* loss_acc = nullptr) {
     Mlp* m = mlp_of(e);
     int nparts = (n + 63) / 64;
     int rc = ensure_parts(e, nparts);
@@ -504,7 +515,8 @@ static int train_batch(Engine& e, const float* x, const float* t, const int32_t*
     }
     hipLaunchKernelGGL(k_mlp_adam, dim3((kMlpParams + 255) / 256), dim3(256), 0, e.stream,
                        m->grad_part, nparts, m->params, m->adam_m, m->adam_v, lr, bc1, bc2,
-                       update, m->grad_sum);
+                       update, m->grad_sum, m->loss_part, 1.0f / (float)(kMlpUsed * n),
+                       loss_acc);
     if (update)
         hipLaunchKernelGGL(k_mlp_refresh, dim3((kMlpParams + 255) / 256), dim3(256), 0,
                            e.stream, m->params, m->wb, m->wtb);
@@ -802,18 +814,22 @@ int32_t cm_mlp_train(void* h, int32_t epochs, int32_t batch, float lr, uint64_t 
         }
         CM_HIP_TRY(*e, hipMemcpyAsync(m->idx, perm.data(), (size_t)n * 4,
                                       hipMemcpyHostToDevice, e->stream));
-        double lsum = 0.0;
+        CM_HIP_TRY(*e, hipMemsetAsync(m->loss_acc, 0, sizeof(double), e->stream));
         int nb = 0;
         for (int b = 0; b < n; b += batch) {
             int cnt = std::min(batch, n - b);
-            float l = 0.0f;
-            int rc = cm::train_batch(*e, m->px, m->pt, m->idx + b, cnt, lr, 1,
-                                     loss_per_epoch ? &l : nullptr);
+            int rc = cm::train_batch(*e, m->px, m->pt, m->idx + b, cnt, lr, 1, nullptr,
+                                     m->loss_acc);
             if (rc != CM_OK) return rc;
-            lsum += l;
             nb++;
         }
-        if (loss_per_epoch) loss_per_epoch[ep] = (float)(lsum / nb);
+        if (loss_per_epoch) {
+            double lsum = 0.0;
+            CM_HIP_TRY(*e, hipMemcpyAsync(&lsum, m->loss_acc, sizeof(double),
+                                          hipMemcpyDeviceToHost, e->stream));
+            CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
+            loss_per_epoch[ep] = (float)(lsum / nb);
+        }
     }
     CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
     return CM_OK;

@@ -10,8 +10,10 @@
 // of Morton-sorted bodies inside one parent cell, so its box never spans a
 // Morton jump into a distant, denser region.
 //
-// Traversal (k_traverse): persistent wavefronts take groups from a device
-// counter.  Each group walks the tree from the root with an LDS stack,
+// Traversal (k_traverse): persistent wavefronts take groups from eight device
+// queues, one per XCD, each an eighth of the groups in body order (so each
+// XCD's L2 serves one compact region).  Each group walks the tree from the
+// root with an LDS stack,
 // 64 nodes per step (fewer when the stack is nearly full, so it never
 // overflows): nodes of zero mass are skipped (BarnesHutBackend.cs:402); a node
 // is accepted as a monopole when d2 > 0 and (4 hL) hL < theta^2 d2, d2 being
@@ -48,10 +50,10 @@ __device__ __forceinline__ double vol_of(const double* lo, const double* hi, dou
     return ((hi[0] - lo[0]) + pad) * ((hi[1] - lo[1]) + pad) * ((hi[2] - lo[2]) + pad);
 }
 
-__device__ __forceinline__ void emit_group(TreeHeader* hdr, int2* groups, int s, int e) {
-    int idx = atomicAdd(&hdr->n_groups, 1);
-    groups[idx] = make_int2(s, e);
-}
+// A group is recorded at its first body (gmark[s] = e); k_group_order then
+// lists the groups in body (Morton) order, so that consecutive groups are
+// spatial neighbours.
+__device__ __forceinline__ void emit_group(int32_t* gmark, int s, int e) { gmark[s] = e; }
 
 // box_split with the three boxes given directly: the bin [s, mid) as lo1/hi1,
 // the next child [mid, e) as lo2/hi2; the whole box is their union.  Node
@@ -73,20 +75,20 @@ __device__ bool box_split_nodes(const double* lo1, const double* hi1,
 
 __global__ void __launch_bounds__(256)
 k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
-         const double* __restrict__ nbox, int gsize, int2* __restrict__ groups) {
+         const double* __restrict__ nbox, int gsize, int32_t* __restrict__ gmark) {
     // a tree that outgrew the node pool is incomplete: no groups, no
     // traversal (the evaluation reports CM_ERR_POOL_EXHAUSTED)
     if (hdr->overflow) return;
     const int nn = hdr->n_nodes;
     for (int k = blockIdx.x * 256 + threadIdx.x; k < nn; k += gridDim.x * 256) {
         int4 nd = info[k];
-        if (k == 0 && nd.w > 0 && nd.w <= gsize) emit_group(hdr, groups, 0, nd.w);
+        if (k == 0 && nd.w > 0 && nd.w <= gsize) emit_group(gmark, 0, nd.w);
         if (nd.w <= gsize) continue;
         if (nd.x < 0) {
             // a deepest-level leaf holding more than gsize bodies: consecutive
             // slices of gsize
             for (int q = nd.z; q < nd.z + nd.w; q += gsize)
-                emit_group(hdr, groups, q, min(q + gsize, nd.z + nd.w));
+                emit_group(gmark, q, min(q + gsize, nd.z + nd.w));
             continue;
         }
         int nch = nd.y & 255;
@@ -112,7 +114,7 @@ k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
                 open = true;
             }
             if (close && bs >= 0) {
-                emit_group(hdr, groups, bs, be);
+                emit_group(gmark, bs, be);
                 bs = -1;
             }
             if (open) {
@@ -120,7 +122,85 @@ k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
                 for (int q = 0; q < 3; q++) { blo[q] = cb[q]; bhi[q] = cb[3 + q]; }
             }
         }
-        if (bs >= 0) emit_group(hdr, groups, bs, be);
+        if (bs >= 0) emit_group(gmark, bs, be);
+    }
+}
+
+// Groups in body order: per tile of kGroupTile bodies count the marks, scan
+// the tile counts, then write each tile's groups at its offset (ballot ranks).
+constexpr int kGroupTile = 1024;
+
+__global__ void __launch_bounds__(256)
+k_fill_i32(int n, int32_t* __restrict__ p) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
+}
+
+__global__ void __launch_bounds__(kGroupTile)
+k_group_count(const TreeHeader* __restrict__ hdr, const int32_t* __restrict__ gmark,
+              int32_t* __restrict__ tcount) {
+    __shared__ int s_w[kGroupTile / kWave];
+    const int n = hdr->n_active;
+    const int i = blockIdx.x * kGroupTile + threadIdx.x;
+    const uint64_t m = __ballot(i < n && gmark[i] > 0);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int w = 0; w < kGroupTile / kWave; w++) s += s_w[w];
+        tcount[blockIdx.x] = s;
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_group_scan(int ntiles, TreeHeader* __restrict__ hdr, int32_t* __restrict__ tcount) {
+    __shared__ int s_w[16];
+    __shared__ int s_carry;
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    if (t == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < ntiles; base += 1024) {
+        const int q = base + t;
+        const int v = q < ntiles ? tcount[q] : 0;
+        int x = v;
+        for (int o = 1; o < kWave; o <<= 1) {
+            int y = __shfl_up(x, o, kWave);
+            if (lane >= o) x += y;
+        }
+        if (lane == kWave - 1) s_w[w] = x;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int k = 0; k < 16; k++) {
+            if (k < w) woff += s_w[k];
+            tot += s_w[k];
+        }
+        const int carry = s_carry;
+        if (q < ntiles) tcount[q] = carry + woff + x - v;
+        __syncthreads();
+        if (t == 0) s_carry = carry + tot;
+        __syncthreads();
+    }
+    if (t == 0) {
+        hdr->n_groups = s_carry;
+        for (int k = 0; k < kXcds; k++) hdr->work_next[k] = 0u;
+    }
+}
+
+__global__ void __launch_bounds__(kGroupTile)
+k_group_emit(const TreeHeader* __restrict__ hdr, const int32_t* __restrict__ gmark,
+             const int32_t* __restrict__ toff, int2* __restrict__ groups) {
+    __shared__ int s_w[kGroupTile / kWave];
+    const int n = hdr->n_active;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int i = blockIdx.x * kGroupTile + threadIdx.x;
+    const int e = i < n ? gmark[i] : 0;
+    const uint64_t m = __ballot(e > 0);
+    if (lane == 0) s_w[w] = __popcll(m);
+    __syncthreads();
+    if (e > 0) {
+        int off = toff[blockIdx.x];
+        for (int k = 0; k < w; k++) off += s_w[k];
+        off += __popcll(m & ((1ull << lane) - 1ull));
+        groups[off] = make_int2(i, e);
     }
 }
 
@@ -244,12 +324,23 @@ k_traverse(TravArgs a) {
     const int reserve = 8 * (a.levels + 2);
     unsigned long long n_cell = 0ull, n_body = 0ull;
 
+    // groups are in body (Morton) order; eighth q of them is queue q, served
+    // first by the wavefronts of XCD q so that each XCD's L2 holds one compact
+    // region of the tree; a wavefront whose queue is empty helps the others
+    const int xcd = blockIdx.x % kXcds;
+    int qi = 0;
     for (;;) {
-        if (lane == 0) s_g = (int)atomicAdd(&a.hdr->work_next, 1u);
+        const int q = (xcd + qi) % kXcds;
+        const int qhi = (int)((long long)(q + 1) * ngroups / kXcds);
+        if (lane == 0)
+            s_g = (int)((long long)q * ngroups / kXcds) + (int)atomicAdd(&a.hdr->work_next[q], 1u);
         __syncthreads();
         const int g = s_g;
         __syncthreads();
-        if (g >= ngroups) break;
+        if (g >= qhi) {
+            if (++qi == kXcds) break;
+            continue;
+        }
         const int2 gr = a.groups[g];
         const int s = gr.x, e = gr.y;
         const int nmem = e - s;
@@ -575,9 +666,17 @@ k_zero_inactive(int n, const TreeHeader* __restrict__ hdr,
 }
 
 int form_groups(Engine& e) {
-    int gb = std::min(1024, std::max(1, (e.count + 255) / 256));
+    const int n = e.count;
+    const int gb = std::min(1024, std::max(1, (n + 255) / 256));
+    const int tiles = std::max(1, (n + kGroupTile - 1) / kGroupTile);
+    hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(256), 0, e.stream, n, e.gmark);
     hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, e.stream, e.hdr, e.nd_info,
-                       e.nd_box, e.cfg.group_size, e.groups);
+                       e.nd_box, e.cfg.group_size, e.gmark);
+    hipLaunchKernelGGL(k_group_count, dim3(tiles), dim3(kGroupTile), 0, e.stream, e.hdr,
+                       e.gmark, e.gtile);
+    hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e.stream, tiles, e.hdr, e.gtile);
+    hipLaunchKernelGGL(k_group_emit, dim3(tiles), dim3(kGroupTile), 0, e.stream, e.hdr,
+                       e.gmark, e.gtile, e.groups);
     CM_HIP_TRY(e, hipGetLastError());
     return CM_OK;
 }

@@ -122,7 +122,7 @@ k_bbox_final(int nblocks, const double* __restrict__ part,
         hdr->n_cell_int = 0ull;
         hdr->n_body_int = 0ull;
         hdr->n_groups = 0;
-        hdr->work_next = 0u;
+        for (int q = 0; q < kXcds; q++) hdr->work_next[q] = 0u;
         nd_info[0] = make_int4(-1, 0, 0, na);
     }
 }
