@@ -90,6 +90,9 @@ def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample, param
         "acc_ms_force": st["acc_ms_force"],
         "evaluations": st["acc_evaluations"],
         "n_nodes": st["n_nodes"],
+        "n_groups": st["n_groups"],
+        "cell_fraction": st["cell_interactions"] / max(1, st["cell_interactions"]
+                                                          + st["body_interactions"]),
     }
 
 
@@ -180,6 +183,91 @@ def copy_calibration(nbytes=1 << 28, reps=5):
     print(json.dumps({"copy_bytes": nbytes, "copies": reps}), flush=True)
 
 
+def disk(args, cmhip):
+    """Collision configuration: N = 262,144 cold thin disk (1 Msun star, a
+    log-uniform 0.5-5 AU, Rayleigh e 0.02 / i 0.01, masses log-uniform 1e-9 -
+    1e-6 Msun, radii from 3 g/cm^3 inflated 100x), FP64 state; a cycle is
+    one Verlet step (FP64 Barnes-Hut, theta 0.5, the bit-exact path), collision
+    detection, MergeOnly resolution and compaction, as NBodySolver runs them."""
+    eps, dt = 1e-4, 1e-3
+    n = args.n if args.n_set else 262144
+    soa = cmhip.generate_disk(n, seed=42, a_min=0.5, a_max=5.0, sigma_e=0.02, sigma_i=0.01,
+                              m_min=1e-9, m_max=1e-6,
+                              density=cmhip.ROCK_DENSITY_INFLATED)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5)
+    be.upload(soa)
+
+    def cycle():
+        be.step(eps, dt, 1)
+        _, ne = be.detect_collisions(0)
+        m = be.resolve_merges()
+        be.compact()
+        return ne, m
+
+    for _ in range(args.warmup):
+        cycle()
+    events = merged = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ne, m = cycle()
+        events += ne
+        merged += m
+    wall = time.perf_counter() - t0
+    line = {"metric": "collision cycles (step + detect + MergeOnly + compact) per second",
+            "value": args.steps / wall, "unit": "cycles/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "disk_collisions_merge", "n_bodies": n, "theta": 0.5,
+                       "softening": eps, "dt": dt, "seed": 42},
+            "body_steps_per_s": n * args.steps / wall, "events": events, "merged": merged,
+            "bodies_left": be.count}
+    print(json.dumps(line), flush=True)
+
+
+def ensemble(args, cmhip):
+    """Batched ensemble: 4096 planetary systems, N uniform in [8, 2048] (a 1 Msun
+    star plus bodies of log-uniform mass 1e-7 - 1e-3 Msun, a 0.1 - 30 AU, Rayleigh
+    e 0.05), direct softened gravity, FP32 split pair terms with FP64 kick-drift,
+    graph-replayed persistent launches (autotuned); per-system energy drift over
+    the timed steps, FP64 path beside it."""
+    eps, dt = 1e-4, 1e-3
+    soa, off = cmhip.generate_ensemble(4096, seed=42)
+    nb = np.diff(off).astype(np.int64)
+    pairs = int((nb * (nb - 1)).sum())
+    res = {}
+    for name, prec in (("fp32_split", cmhip.PREC_FP32_SPLIT), ("fp64", cmhip.PREC_FP64)):
+        ens = cmhip.HipEnsemble(off, precision=prec, threads=0, steps_per_launch=100)
+        ens.upload(soa)
+        ens.step(eps, dt, args.warmup)
+        e0 = ens.energy(eps)
+        t0 = time.perf_counter()
+        ens.step(eps, dt, args.steps)
+        wall = time.perf_counter() - t0
+        e1 = ens.energy(eps)
+        drift = np.abs((e1 - e0) / e0)
+        st = ens.stats()
+        res[name] = {"ms_per_step": 1e3 * wall / args.steps,
+                     "interactions_per_s": pairs * args.steps / wall,
+                     "tflops_20_per_pair": 20 * pairs * args.steps / wall / 1e12,
+                     "energy_drift_median": float(np.median(drift)),
+                     "energy_drift_max": float(drift.max()),
+                     "threads": st["threads"], "cost_order": st["tuned_cost_order"]}
+        log(json.dumps({name: res[name]}))
+    f = res["fp32_split"]
+    line = {"metric": "ensemble body-steps/s (4096 systems, direct)", "unit": "body-steps/s",
+            "value": float(nb.sum()) * 1e3 / f["ms_per_step"], "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": f["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32-split (f64 state)", "data": "synthetic",
+            "config": {"workload": "planetary_ensemble_direct", "systems": 4096,
+                       "bodies": int(nb.sum()), "pair_terms_per_step": pairs,
+                       "softening": eps, "dt": dt, "seed": 42},
+            "fp32_split": f, "fp64": res["fp64"],
+            "speedup_vs_fp64": res["fp64"]["ms_per_step"] / f["ms_per_step"]}
+    print(json.dumps(line), flush=True)
+
+
 def merger(args, cmhip):
     eps, dt = 1e-4, 1e-3
     n = args.n if args.n_set else 1 << 22
@@ -230,7 +318,8 @@ def main():
                     help="also time the fast path at this theta (0 to skip)")
     ap.add_argument("--theta-learned", type=float, default=0.75,
                     help="also time the learned correction at this theta (0 to skip)")
-    ap.add_argument("--workload", choices=("plummer", "merger"), default="plummer")
+    ap.add_argument("--workload", choices=("plummer", "merger", "disk", "ensemble"),
+                    default="plummer")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fast-only", action="store_true",
@@ -247,6 +336,12 @@ def main():
         return
     args.n_set = args.n is not None
     import cmhip
+    if args.workload == "ensemble":
+        ensemble(args, cmhip)
+        return
+    if args.workload == "disk":
+        disk(args, cmhip)
+        return
     if args.workload == "merger":
         merger(args, cmhip)
         return
@@ -320,6 +415,7 @@ def main():
         "force_err_median": fast["force_err_median"],
         "force_err_max": fast["force_err_max"],
         "interactions_per_body": fast["interactions_per_body"],
+        "cell_fraction": fast["cell_fraction"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
                      "traffic": traffic_from_profile(), "kernel": "k_traverse<FP32_SPLIT>",
