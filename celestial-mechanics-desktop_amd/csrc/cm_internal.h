@@ -15,7 +15,7 @@ namespace cm {
 
 constexpr int kWave = 64;
 constexpr int kMaxLevels = 21;          // 63-bit keys
-constexpr int kStackCap = 2048;         // per-wave LDS traversal stack (ints)
+constexpr int kStackCap = 1536;         // per-wave LDS traversal stack (ints)
 constexpr int kSortTile = 2048;         // keys per radix-sort tile
 constexpr int kSortThreads = 256;
 constexpr int kTreeTile = 1024;         // bodies per tree-build tile

@@ -67,11 +67,23 @@ __device__ __forceinline__ void pair_acc(double xi, double yi, double zi, double
     }
 }
 
+// FP32 split pair term: offset formed in double, the rest in float.
+__device__ __forceinline__ void pair_f(const double4 p, const double4 q, float e2, float& fx,
+                                       float& fy, float& fz) {
+    const float x = (float)(p.x - q.x), y = (float)(p.y - q.y), z = (float)(p.z - q.z);
+    const float r2 = __builtin_fmaf(x, x, __builtin_fmaf(y, y, __builtin_fmaf(z, z, e2)));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float c = (float)q.w * (inv * inv * inv);
+    fx = __builtin_fmaf(-c, x, fx);
+    fy = __builtin_fmaf(-c, y, fy);
+    fz = __builtin_fmaf(-c, z, fz);
+}
+
 template <int PREC, int NT>
 __global__ void __launch_bounds__(NT)
 k_ens_run(EnsArgs a) {
-    __shared__ double s_x[kEnsMaxBodies], s_y[kEnsMaxBodies], s_z[kEnsMaxBodies],
-        s_m[kEnsMaxBodies];
+    // the system's positions and masses (x, y, z, m; m = 0 for inactive bodies)
+    __shared__ double4 s_p[kEnsMaxBodies];
     __shared__ uint8_t s_a[kEnsMaxBodies];
     __shared__ int s_sys;
     const int t = threadIdx.x;
@@ -87,9 +99,21 @@ k_ens_run(EnsArgs a) {
         const int b0 = a.offsets[sys];
         const int nb = a.offsets[sys + 1] - b0;
         for (int i = t; i < nb; i += NT) {
-            s_m[i] = a.m[b0 + i];
-            s_a[i] = a.act[b0 + i];
+            const uint8_t act = a.act[b0 + i];
+            s_p[i].w = act ? a.m[b0 + i] : 0.0;
+            s_a[i] = act;
         }
+        // final kick and rotate of target i from its acceleration
+        auto finish = [&](int i, double sx, double sy, double sz) {
+            const int g = b0 + i;
+            a.ax[g] = sx; a.ay[g] = sy; a.az[g] = sz;
+            if (s_a[i]) {
+                a.vx[g] = a.vx[g] + half * sx;
+                a.vy[g] = a.vy[g] + half * sy;
+                a.vz[g] = a.vz[g] + half * sz;
+            }
+            a.oax[g] = sx; a.oay[g] = sy; a.oaz[g] = sz;
+        };
         for (int step = 0; step < a.steps; step++) {
             // phases 1-2: half kick with the old acceleration, drift
             for (int i = t; i < nb; i += NT) {
@@ -103,33 +127,62 @@ k_ens_run(EnsArgs a) {
                     x = x + a.dt * vx; y = y + a.dt * vy; z = z + a.dt * vz;
                     a.px[g] = x; a.py[g] = y; a.pz[g] = z;
                 }
-                s_x[i] = x; s_y[i] = y; s_z[i] = z;
+                s_p[i].x = x; s_p[i].y = y; s_p[i].z = z;
             }
             __syncthreads();
-            // phase 3: forces (j ascending, self and inactive skipped)
-            for (int i = t; i < nb; i += NT) {
-                int g = b0 + i;
-                double sx = 0.0, sy = 0.0, sz = 0.0;
-                float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-                if (s_a[i]) {
-                    const double xi = s_x[i], yi = s_y[i], zi = s_z[i];
-                    for (int j = 0; j < nb; j++) {
-                        if (j == i || !s_a[j]) continue;
-                        pair_acc<PREC>(xi, yi, zi, s_x[j], s_y[j], s_z[j], s_m[j], a.eps2,
-                                       sx, sy, sz, fx, fy, fz);
+            // phase 3: forces, then phases 4-5 (second half kick, rotate)
+            if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                // two targets per thread share every source read; inactive
+                // sources carry zero mass and, with eps2 > 0, the self term
+                // is exactly zero, so neither needs a branch (adding an exact
+                // zero leaves each float sum unchanged)
+                const float e2 = (float)a.eps2;
+                const bool skip_self = !(a.eps2 > 0.0);
+                for (int i0 = t; i0 < nb; i0 += 2 * NT) {
+                    const int i1 = i0 + NT;
+                    const bool has1 = i1 < nb;
+                    const double4 p0 = s_p[i0];
+                    const double4 p1 = s_p[has1 ? i1 : i0];
+                    float ax0 = 0.0f, ay0 = 0.0f, az0 = 0.0f, ax1 = 0.0f, ay1 = 0.0f, az1 = 0.0f;
+                    if (!skip_self) {
+                        for (int j = 0; j < nb; j++) {
+                            const double4 qj = s_p[j];
+                            pair_f(p0, qj, e2, ax0, ay0, az0);
+                            pair_f(p1, qj, e2, ax1, ay1, az1);
+                        }
+                    } else {
+                        for (int j = 0; j < nb; j++) {
+                            const double4 qj = s_p[j];
+                            if (j != i0) pair_f(p0, qj, e2, ax0, ay0, az0);
+                            if (j != i1) pair_f(p1, qj, e2, ax1, ay1, az1);
+                        }
+                    }
+                    const bool act0 = s_a[i0] != 0;
+                    finish(i0, act0 ? (double)ax0 : 0.0, act0 ? (double)ay0 : 0.0,
+                           act0 ? (double)az0 : 0.0);
+                    if (has1) {
+                        const bool act1 = s_a[i1] != 0;
+                        finish(i1, act1 ? (double)ax1 : 0.0, act1 ? (double)ay1 : 0.0,
+                               act1 ? (double)az1 : 0.0);
                     }
                 }
-                if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                    sx = (double)fx; sy = (double)fy; sz = (double)fz;
+            } else {
+                // FP64: CpuParallelBackend's sweep (j ascending, self and
+                // inactive skipped), bit for bit
+                for (int i = t; i < nb; i += NT) {
+                    double sx = 0.0, sy = 0.0, sz = 0.0;
+                    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+                    if (s_a[i]) {
+                        const double4 pi = s_p[i];
+                        for (int j = 0; j < nb; j++) {
+                            if (j == i || !s_a[j]) continue;
+                            const double4 qj = s_p[j];
+                            pair_acc<PREC>(pi.x, pi.y, pi.z, qj.x, qj.y, qj.z, qj.w, a.eps2,
+                                           sx, sy, sz, fx, fy, fz);
+                        }
+                    }
+                    finish(i, sx, sy, sz);
                 }
-                a.ax[g] = sx; a.ay[g] = sy; a.az[g] = sz;
-                // phases 4-5: second half kick, rotate
-                if (s_a[i]) {
-                    a.vx[g] = a.vx[g] + half * sx;
-                    a.vy[g] = a.vy[g] + half * sy;
-                    a.vz[g] = a.vz[g] + half * sz;
-                }
-                a.oax[g] = sx; a.oay[g] = sy; a.oaz[g] = sz;
             }
             __syncthreads();
         }

@@ -241,7 +241,7 @@ __global__ void __launch_bounds__(kWave)
 k_mlp_collect(const TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
               const double4* __restrict__ cm, const int2* __restrict__ groups,
               const int32_t* __restrict__ gsel, int nsel, const double* __restrict__ sorted,
-              int cap, double theta2, int shell, int2* __restrict__ pairs,
+              int cap, int levels, double theta2, int shell, int2* __restrict__ pairs,
               double4* __restrict__ pair_c, int pair_cap, int32_t* __restrict__ counter) {
     __shared__ int s_stack[kStackCap];
     __shared__ int s_base;
@@ -270,11 +270,14 @@ k_mlp_collect(const TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
         }
         const double4 centre = make_double4((lo0 + hi0) * 0.5, (lo1 + hi1) * 0.5,
                                             (lo2 + hi2) * 0.5, 0.0);
+        // a forced single pop can still push 8 children per level below the
+        // limit, so one path's worth of slots stays free
+        const int reserve = 8 * (levels + 2);
         int S = 1;
         if (lane == 0) s_stack[0] = 0;
         __syncthreads();
         while (S > 0) {
-            int lim = (kStackCap - S) / 7;
+            int lim = (kStackCap - reserve - S) / 7;
             if (lim < 1) lim = 1;
             int take = S < kWave ? S : kWave;
             if (take > lim) take = lim;
@@ -709,7 +712,8 @@ int32_t cm_mlp_collect(void* h, double softening, int32_t max_groups, uint64_t s
     int nsel = (int)sel.size();
     hipLaunchKernelGGL(cm::k_mlp_collect, dim3(std::min(nsel, 4096)), dim3(cm::kWave), 0,
                        e->stream, e->hdr, e->nd_info, e->nd_cm, e->groups, m->gsel, nsel,
-                       e->sorted, e->capacity, th * th, e->cfg.shell_theorem, m->pairs,
+                       e->sorted, e->capacity, e->cfg.morton_levels, th * th,
+                       e->cfg.shell_theorem, m->pairs,
                        m->pair_c, max_pairs, m->counter);
     CM_HIP_TRY(*e, hipGetLastError());
     int32_t found = 0;

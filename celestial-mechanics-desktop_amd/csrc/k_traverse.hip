@@ -459,6 +459,10 @@ k_traverse(TravArgs a) {
         };
 
         auto flush = [&](int cnt, bool cells) {
+            if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                // pad to a multiple of four with massless far entries
+                if (lane >= cnt) st.v[lane] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+            }
             if (lane < cnt) {
                 if (cells) {
                     double4 v = a.cm[s_cell[lane]];
@@ -493,9 +497,9 @@ k_traverse(TravArgs a) {
                     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
                     const float e2 = (float)a.eps2;
                     if (cells || (!shell && e2 > 0.0f)) {
-                        // with eps2 > 0 the self term (dx = 0) is exactly zero
-                        for (int j = 0; j < cnt; j++) {
-                            float4 v = st.v[j];
+                        // with eps2 > 0 the self term (dx = 0) is exactly zero;
+                        // entries past cnt are massless and far away
+                        auto term = [&](const float4 v) {
                             float dx = txf - v.x, dy = tyf - v.y, dz = tzf - v.z;
                             float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
                             float inv = __builtin_amdgcn_rsqf(r2);
@@ -503,6 +507,16 @@ k_traverse(TravArgs a) {
                             gx = __builtin_fmaf(-f, dx, gx);
                             gy = __builtin_fmaf(-f, dy, gy);
                             gz = __builtin_fmaf(-f, dz, gz);
+                        };
+                        const int cnt4 = (cnt + 3) & ~3;
+                        float4 v0 = st.v[0], v1 = st.v[1], v2 = st.v[2], v3 = st.v[3];
+                        for (int j = 0; j < cnt4; j += 4) {
+                            // next four entries are loaded while these four compute
+                            const int jn = (j + 4) & (kWave - 1);
+                            const float4 n0 = st.v[jn], n1 = st.v[jn + 1], n2 = st.v[jn + 2],
+                                         n3 = st.v[jn + 3];
+                            term(v0); term(v1); term(v2); term(v3);
+                            v0 = n0; v1 = n1; v2 = n2; v3 = n3;
                         }
                     } else {
                         for (int j = 0; j < cnt; j++) {

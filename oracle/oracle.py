@@ -251,7 +251,7 @@ def tree_build(s, levels, leaf_cap):
 
 
 def bh_tree(s, eps, theta, levels=10, leaf_cap=16, group_size=64, shell=False,
-            split_depth=-3, lifo=2048, nthreads=0):
+            split_depth=-3, lifo=1536, nthreads=0):
     n = len(s["px"])
     ax, ay, az = (np.zeros(n) for _ in range(3))
     nc = np.zeros(n, dtype=np.int32)

@@ -88,7 +88,7 @@ def test_barnes_hut_fp64_bit_exact(plummer, levels, leaf, gsize):
                                  leaf_capacity=leaf, group_size=gsize)
     be.compute_forces(soa, EPS)
     ref, st = O.bh_tree(s, EPS, 0.5, levels=levels, leaf_cap=leaf, group_size=gsize,
-                        split_depth=-3, lifo=2048, nthreads=8)
+                        split_depth=-3, lifo=1536, nthreads=8)
     _eq(_acc(soa), ref)
     gs = be.stats()
     assert gs["n_active"] == int(s["active"].sum())
@@ -106,7 +106,7 @@ def test_barnes_hut_fp32_split_vs_fp64(plummer):
     be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP32_SPLIT, theta=0.5)
     be.compute_forces(soa, EPS)
     ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=-3,
-                       lifo=2048, nthreads=8)
+                       lifo=1536, nthreads=8)
     r = _rel(_acc(soa), ref)
     assert np.median(r) <= 1e-6, np.median(r)
     assert r.max() <= 1e-3, r.max()
@@ -120,7 +120,7 @@ def test_barnes_hut_fp64_fast_vs_exact(plummer):
     be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64_FAST, theta=0.5)
     be.compute_forces(soa, EPS)
     ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=-3,
-                       lifo=2048, nthreads=8)
+                       lifo=1536, nthreads=8)
     assert _rel(_acc(soa), ref).max() <= 1e-13
 
 

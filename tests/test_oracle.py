@@ -286,7 +286,7 @@ def test_group_traversal_reduces_to_reference_bh():
     s = state(p[:, 0], p[:, 1], p[:, 2], rng.uniform(0.5, 2, len(p)))
     ref, _ = O.bh_csharp(s, 1e-3, 0.5)
     got, st = O.bh_tree(s, 1e-3, 0.5, levels=21, leaf_cap=1, group_size=1, split_depth=0,
-                        lifo=2048)
+                        lifo=1536)
     assert rel(got, ref).max() < 1e-12
 
 
@@ -296,7 +296,7 @@ def test_group_traversal_converges_to_direct():
     errs = []
     for th in (0.2, 0.5, 0.9):
         a, _ = O.bh_tree(s, 1e-3, th, levels=21, leaf_cap=8, group_size=64, split_depth=-3,
-                         lifo=2048)
+                         lifo=1536)
         errs.append(np.median(rel(a, ref)))
     assert errs[0] < 1e-4 and errs[0] < errs[1] < errs[2]
 
@@ -304,7 +304,7 @@ def test_group_traversal_converges_to_direct():
 def test_group_traversal_stack_bounded():
     s = random_state(20000, seed=14, spread=3.0)
     _, st = O.bh_tree(s, 1e-3, 0.5, levels=10, leaf_cap=16, group_size=64, split_depth=-3,
-                      lifo=2048)
+                      lifo=1536)
     assert st["max_queue"] <= 2048
 
 
