@@ -268,6 +268,119 @@ def ensemble(args, cmhip):
     print(json.dumps(line), flush=True)
 
 
+PEAK_FP64_VALU_TFLOPS = 78.6   # MI355X published FP64 vector peak
+FLOPS_PER_PN_PAIR = 81          # PostNewtonian1Correction pair term, sqrt and divide as 1 each
+
+
+def _time_forces(be, reps):
+    be.compute_forces_resident(1e-4)
+    be.stats()                               # synchronises
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        be.compute_forces_resident(1e-4)
+    be.stats()
+    return (time.perf_counter() - t0) / reps
+
+
+def extensions(args, cmhip):
+    """The rows that widen the hot path, at their measurement sizes.
+    Direct FP64 sum at N = 65,536 (Plummer, masses log-uniform in
+    [1e-9, 1], 16 body types) with the engine's global, per-type and adaptive
+    softening; the 1PN decorator (k_pn) on top of it, velocities ~1 % of c;
+    the Yoshida-4 step on the headline Barnes-Hut workload (N = 2^20, FP32
+    split, theta = 0.5) beside the Verlet step.  Kernel times are wall-clock
+    differences over `reps` device-resident evaluations."""
+    import oracle as O   # the CPU baselines only
+    eps = 1e-4
+    n = args.n or 65536
+    reps = 10
+    soa = cmhip.generate_plummer(n, 1.0, 1.0, seed=42, mass_fraction_cut=0.999)
+    rng = np.random.default_rng(42)
+    soa.mass[:n] = 10.0 ** rng.uniform(-9, 0, n)
+    soa.body_type[:n] = rng.integers(0, 16, n)
+    c = 2.998e8 * 5.0226e6 / 1.496e11   # PhysicalConstants.C_Sim
+    for f in ("vel_x", "vel_y", "vel_z"):
+        getattr(soa, f)[:n] = rng.normal(0, 0.01 * c, n)
+    table = rng.uniform(1e-4, 1e-2, 16)
+    be = cmhip.HipPhysicsBackend(n, kind=cmhip.BACKEND_DIRECT, precision=cmhip.PREC_FP64)
+    be.upload(soa)
+    pairs = float(n) * (n - 1)
+    res = {}
+    for name, mode in (("global", cmhip.SOFTENING_GLOBAL),
+                       ("per_type", cmhip.SOFTENING_PER_TYPE),
+                       ("adaptive", cmhip.SOFTENING_ADAPTIVE)):
+        be.set_softening(mode, type_softening=table, adaptive_scale=0.01)
+        t = _time_forces(be, reps)
+        res[name] = {"ms": 1e3 * t, "pairs_per_s": pairs / t}
+        log(json.dumps({name: res[name]}))
+    be.set_softening(cmhip.SOFTENING_GLOBAL)
+    be.set_post_newtonian(True)
+    t_pn = _time_forces(be, reps) - res["global"]["ms"] * 1e-3
+    _, nw = be.pn_warnings(1)
+    be.close()
+    pn_tflops = FLOPS_PER_PN_PAIR * pairs / t_pn / 1e12
+    res["pn"] = {"ms_k_pn": 1e3 * t_pn, "pairs_per_s": pairs / t_pn, "tflops": pn_tflops,
+                 "warnings": nw,
+                 "roofline": {"bound": "valu", "achieved": pn_tflops,
+                              "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+                              "frac": pn_tflops / PEAK_FP64_VALU_TFLOPS}}
+    log(json.dumps({"pn": res["pn"]}))
+    # CPU: the oracle's restatements (the reference's loops in C), one thread,
+    # on the first 4096 bodies of the same state
+    k = 4096
+    st = {"px": soa.pos_x[:k].copy(), "py": soa.pos_y[:k].copy(), "pz": soa.pos_z[:k].copy(),
+          "vx": soa.vel_x[:k].copy(), "vy": soa.vel_y[:k].copy(), "vz": soa.vel_z[:k].copy(),
+          "mass": soa.mass[:k].copy(), "active": np.ones(k, dtype=np.uint8),
+          "body_type": soa.body_type[:k].copy()}
+    z = np.zeros(k)
+    t0 = time.perf_counter()
+    O.pn_apply(st, eps, z, z, z)
+    t_cpu_pn = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.direct_softened(st, 2, scale=0.01)
+    t_cpu_ad = time.perf_counter() - t0
+    cpu_pairs = float(k) * (k - 1)
+    res["pn"]["cpu_baseline"] = {"value": cpu_pairs / t_cpu_pn, "unit": "pairs/s", "cores": 1,
+                                 "kind": "port",
+                                 "sample": "PostNewtonian1Correction.ApplyCorrections on 4096 "
+                                           "bodies of the same state"}
+    res["adaptive"]["cpu_baseline"] = {"value": cpu_pairs / t_cpu_ad, "unit": "pairs/s",
+                                       "cores": 1, "kind": "port",
+                                       "sample": "Engine::compute_cpu_brute_forces (adaptive) "
+                                                 "on 4096 bodies of the same state"}
+    # Yoshida-4 vs Verlet on the headline Barnes-Hut workload
+    nb = 1 << 20
+    big = cmhip.generate_plummer(nb, 1.0, 1.0, seed=42, mass_fraction_cut=0.999)
+    steps = max(2, args.steps // 4)
+    yo = {}
+    for name in ("verlet", "yoshida4"):
+        b = make_backend(cmhip, nb, cmhip.PREC_FP32_SPLIT, 0.5)
+        b.upload(big)
+        fn = b.step if name == "verlet" else b.yoshida_step
+        fn(eps, 1e-3, 1)
+        b.stats()
+        t0 = time.perf_counter()
+        fn(eps, 1e-3, steps)
+        b.stats()
+        t = (time.perf_counter() - t0) / steps
+        yo[name] = {"ms_per_step": 1e3 * t, "body_steps_per_s": nb / t}
+        b.close()
+    yo["yoshida4"]["force_evaluations_per_step"] = 3
+    res["yoshida4_bh"] = yo
+    log(json.dumps({"yoshida4_bh": yo}))
+    line = {"metric": "1PN pair terms/s, FP64 direct sum at N=65536 (k_pn)",
+            "value": res["pn"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
+            "steps": reps, "warmup": 1, "ms_per_step": res["pn"]["ms_k_pn"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "pn_softening_yoshida", "n_bodies": n, "softening": eps,
+                       "velocity_sigma_c": 0.01, "seed": 42},
+            "roofline": res["pn"]["roofline"], "cpu_baseline": res["pn"]["cpu_baseline"],
+            "softening_modes": {k: res[k] for k in ("global", "per_type", "adaptive")},
+            "yoshida4_bh": yo}
+    print(json.dumps(line), flush=True)
+
+
 def merger(args, cmhip):
     eps, dt = 1e-4, 1e-3
     n = args.n if args.n_set else 1 << 22
@@ -318,7 +431,8 @@ def main():
                     help="also time the fast path at this theta (0 to skip)")
     ap.add_argument("--theta-learned", type=float, default=0.75,
                     help="also time the learned correction at this theta (0 to skip)")
-    ap.add_argument("--workload", choices=("plummer", "merger", "disk", "ensemble"),
+    ap.add_argument("--workload", choices=("plummer", "merger", "disk", "ensemble",
+                                           "extensions"),
                     default="plummer")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -344,6 +458,10 @@ def main():
         return
     if args.workload == "merger":
         merger(args, cmhip)
+        return
+    if args.workload == "extensions":
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        extensions(args, cmhip)
         return
     if args.fast_only:
         args.no_fp64 = True

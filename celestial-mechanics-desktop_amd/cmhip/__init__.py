@@ -55,7 +55,14 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_ensemble_upload", "cm_ensemble_download", "cm_ensemble_step",
            "cm_ensemble_energy", "cm_ensemble_get_stats", "cm_generate_ensemble",
            "cm_mlp_init", "cm_mlp_set_params", "cm_mlp_get_params", "cm_mlp_forward",
-           "cm_mlp_gradient", "cm_mlp_collect", "cm_mlp_get_pairs", "cm_mlp_train")
+           "cm_mlp_gradient", "cm_mlp_collect", "cm_mlp_get_pairs", "cm_mlp_train",
+           "cm_default_pn_config", "cm_set_post_newtonian", "cm_pn_warnings",
+           "cm_default_softening_config", "cm_set_softening", "cm_yoshida4_step",
+           "cm_default_collision_config", "cm_configure_collisions", "cm_resolve_collisions")
+
+COLLISION_MERGE_ONLY, COLLISION_BOUNCE_ONLY, COLLISION_REALISTIC = 0, 1, 2
+
+SOFTENING_GLOBAL, SOFTENING_PER_TYPE, SOFTENING_ADAPTIVE = 0, 1, 2
 
 MLP_IN, MLP_OUT, MLP_USED, MLP_PARAMS = 16, 16, 8, 10448
 
@@ -76,6 +83,30 @@ class BackendConfig(C.Structure):
                 ("shell_theorem", C.c_int32), ("morton_levels", C.c_int32),
                 ("theta", C.c_double), ("leaf_capacity", C.c_int32),
                 ("group_size", C.c_int32), ("learned_correction", C.c_int32)]
+
+
+class PnConfig(C.Structure):
+    _fields_ = [("enabled", C.c_int32), ("max_velocity_fraction_c", C.c_double),
+                ("schwarzschild_warning_factor", C.c_double)]
+
+
+class PnWarning(C.Structure):
+    _fields_ = [("body", C.c_int32), ("other", C.c_int32), ("distance", C.c_double),
+                ("schwarzschild_radius", C.c_double)]
+
+
+class SofteningConfig(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("type_softening", C.c_double * 16),
+                ("adaptive_scale", C.c_double), ("adaptive_min", C.c_double)]
+
+
+class CollisionConfig(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("default_restitution", C.c_double),
+                ("fragmentation_specific_energy_threshold", C.c_double),
+                ("fragmentation_mass_loss_cap", C.c_double),
+                ("capture_velocity_factor", C.c_double),
+                ("promote_compact_remnants", C.c_int32),
+                ("max_merges_per_frame", C.c_int32), ("max_merges_per_body", C.c_int32)]
 
 
 class SoAPtrs(C.Structure):
@@ -172,6 +203,15 @@ def load_library():
         "cm_mlp_collect": ([vp, f64, i32, C.c_uint64, i32, C.POINTER(i32)], i32),
         "cm_mlp_get_pairs": ([vp, vp, vp, vp, vp, i32], i32),
         "cm_mlp_train": ([vp, i32, i32, C.c_float, C.c_uint64, vp], i32),
+        "cm_default_pn_config": ([C.POINTER(PnConfig)], i32),
+        "cm_set_post_newtonian": ([vp, C.POINTER(PnConfig)], i32),
+        "cm_pn_warnings": ([vp, vp, i32, C.POINTER(i32)], i32),
+        "cm_default_softening_config": ([C.POINTER(SofteningConfig)], i32),
+        "cm_set_softening": ([vp, C.POINTER(SofteningConfig)], i32),
+        "cm_yoshida4_step": ([vp, f64, f64, i32], i32),
+        "cm_default_collision_config": ([C.POINTER(CollisionConfig)], i32),
+        "cm_configure_collisions": ([vp, C.POINTER(CollisionConfig)], i32),
+        "cm_resolve_collisions": ([vp, C.POINTER(i32)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -382,6 +422,42 @@ class HipPhysicsBackend:
         self._check(self._L.cm_verlet_step(self._h, float(softening), float(dt), int(n_steps)),
                     "cm_verlet_step")
 
+    def yoshida_step(self, softening, dt, n_steps=1):
+        """n_steps of the engine's Yoshida4 integrator (three Verlet sub-steps)."""
+        self._check(self._L.cm_yoshida4_step(self._h, float(softening), float(dt),
+                                             int(n_steps)), "cm_yoshida4_step")
+
+    # -- PostNewtonianBackend decorator and the engine's softening modes -----
+    def set_post_newtonian(self, enabled=True, max_velocity_fraction_c=0.3,
+                           schwarzschild_warning_factor=3.0):
+        c = PnConfig()
+        self._L.cm_default_pn_config(C.byref(c))
+        c.enabled = int(bool(enabled))
+        c.max_velocity_fraction_c = float(max_velocity_fraction_c)
+        c.schwarzschild_warning_factor = float(schwarzschild_warning_factor)
+        self._check(self._L.cm_set_post_newtonian(self._h, C.byref(c)), "cm_set_post_newtonian")
+
+    def pn_warnings(self, cap=4096):
+        """Strong-field pairs skipped by the last evaluation: (list of
+        (body, other, distance, rs) in (body, other) order, total count)."""
+        buf = (PnWarning * max(1, cap))()
+        n = C.c_int32()
+        self._check(self._L.cm_pn_warnings(self._h, buf, int(cap), C.byref(n)), "cm_pn_warnings")
+        k = min(n.value, cap, 4096)
+        return [(w.body, w.other, w.distance, w.schwarzschild_radius) for w in buf[:k]], n.value
+
+    def set_softening(self, mode=SOFTENING_GLOBAL, type_softening=None, adaptive_scale=0.01,
+                      adaptive_min=1e-8):
+        c = SofteningConfig()
+        self._L.cm_default_softening_config(C.byref(c))
+        c.mode = int(mode)
+        if type_softening is not None:
+            for k, v in enumerate(type_softening):
+                c.type_softening[k] = float(v)
+        c.adaptive_scale = float(adaptive_scale)
+        c.adaptive_min = float(adaptive_min)
+        self._check(self._L.cm_set_softening(self._h, C.byref(c)), "cm_set_softening")
+
     def stats(self):
         s = ForceStats()
         self._check(self._L.cm_get_force_stats(self._h, C.byref(s)), "cm_get_force_stats")
@@ -417,6 +493,29 @@ class HipPhysicsBackend:
     def resolve_merges(self):
         n = C.c_int32()
         self._check(self._L.cm_resolve_merges(self._h, C.byref(n)), "cm_resolve_merges")
+        return n.value
+
+    def configure_collisions(self, mode=COLLISION_MERGE_ONLY, default_restitution=0.15,
+                             fragmentation_specific_energy_threshold=0.6,
+                             fragmentation_mass_loss_cap=0.3, capture_velocity_factor=0.9,
+                             promote_compact_remnants=False, max_merges_per_frame=0,
+                             max_merges_per_body=0):
+        c = CollisionConfig()
+        self._L.cm_default_collision_config(C.byref(c))
+        c.mode = int(mode)
+        c.default_restitution = float(default_restitution)
+        c.fragmentation_specific_energy_threshold = float(fragmentation_specific_energy_threshold)
+        c.fragmentation_mass_loss_cap = float(fragmentation_mass_loss_cap)
+        c.capture_velocity_factor = float(capture_velocity_factor)
+        c.promote_compact_remnants = int(bool(promote_compact_remnants))
+        c.max_merges_per_frame = int(max_merges_per_frame)
+        c.max_merges_per_body = int(max_merges_per_body)
+        self._check(self._L.cm_configure_collisions(self._h, C.byref(c)),
+                    "cm_configure_collisions")
+
+    def resolve_collisions(self):
+        n = C.c_int32()
+        self._check(self._L.cm_resolve_collisions(self._h, C.byref(n)), "cm_resolve_collisions")
         return n.value
 
     def compact(self):
@@ -517,15 +616,31 @@ class CollisionDetector:
 
 
 class CollisionResolver:
-    """CollisionResolver.Resolve in CollisionMode.MergeOnly
-    (Collisions/CollisionResolver.cs:58-130) over the events of the last
+    """CollisionResolver (Collisions/CollisionResolver.cs:23-247): Configure
+    sets the mode and parameters, Resolve applies the events of the last
     detection on the same backend; last_collision_count mirrors
     LastCollisionCount."""
 
     last_collision_count = 0
 
-    def resolve(self, backend):
-        self.last_collision_count = backend.resolve_merges()
+    def __init__(self):
+        self._cfg = {}
+
+    def configure(self, mode, default_restitution, fragmentation_specific_energy_threshold,
+                  fragmentation_mass_loss_cap, capture_velocity_factor):
+        self._cfg = dict(mode=mode, default_restitution=default_restitution,
+                         fragmentation_specific_energy_threshold=
+                         fragmentation_specific_energy_threshold,
+                         fragmentation_mass_loss_cap=fragmentation_mass_loss_cap,
+                         capture_velocity_factor=capture_velocity_factor)
+
+    def resolve(self, backend, promote_compact_remnants=False):
+        if self._cfg or promote_compact_remnants:
+            backend.configure_collisions(promote_compact_remnants=promote_compact_remnants,
+                                         **self._cfg)
+            self.last_collision_count = backend.resolve_collisions()
+        else:
+            self.last_collision_count = backend.resolve_merges()
         return self.last_collision_count
 
 

@@ -96,6 +96,12 @@ struct Engine {
     // collisions / compaction scratch
     void* coll = nullptr;
 
+    // 1PN decorator (k_pn.hip) and direct-sum softening modes (k_direct.hip)
+    cm_pn_config pn{};
+    void* pn_buf = nullptr;
+    cm_softening_config soft{};
+    cm_collision_config coll_cfg{};
+
     // diagnostics scratch
     double* red = nullptr;             // reduction partials
     double* sample = nullptr;          // direct-sample partials
@@ -127,12 +133,14 @@ int launch_direct_sample(Engine& e, double softening, const int32_t* targets,
                          int32_t nt, double* ax, double* ay, double* az);
 int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out);
 int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne);
-int collide_resolve(Engine& e, int32_t* merged);
+int collide_resolve(Engine& e, const cm_collision_config& cfg, int32_t* merged);
 int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index);
 void collide_free(Engine& e);
 int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out);
 int form_groups(Engine& e);
 void mlp_free(Engine& e);
+int launch_pn(Engine& e, double softening);
+void pn_free(Engine& e);
 bool mlp_ready(const Engine& e);
 const uint16_t* mlp_weights(const Engine& e);   // bf16 [out][in] per layer
 const float* mlp_params(const Engine& e);       // FP32 master (biases)

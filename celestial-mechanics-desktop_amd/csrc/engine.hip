@@ -132,9 +132,21 @@ int launch_kick_rotate(Engine& e, double dt) {
     return CM_OK;
 }
 
+// The configured backend's Newtonian forces, then the 1PN decorator when it
+// is on (PostNewtonianBackend.ComputeForces, Solvers/PostNewtonianBackend.cs:33-43).
 static int compute_forces(Engine& e, double softening) {
-    if (e.cfg.kind == CM_BACKEND_BARNES_HUT) return launch_barnes_hut(e, softening);
-    return launch_direct(e, softening);
+    int rc = e.cfg.kind == CM_BACKEND_BARNES_HUT ? launch_barnes_hut(e, softening)
+                                                 : launch_direct(e, softening);
+    if (rc != CM_OK) return rc;
+    return launch_pn(e, softening);
+}
+
+// One SoAVerletIntegrator.Step (Integrators/SoAVerletIntegrator.cs:48-115).
+static int verlet_once(Engine& e, double softening, double dt) {
+    int rc = launch_kick_drift(e, dt);
+    if (rc == CM_OK) rc = compute_forces(e, softening);
+    if (rc == CM_OK) rc = launch_kick_rotate(e, dt);
+    return rc;
 }
 
 static double* const* field_ptrs(const cm_soa_ptrs* p, double* out[NF64]) {
@@ -157,6 +169,7 @@ static void free_all(Engine& e) {
     if (e.hdr_host) (void)hipHostFree(e.hdr_host);
     collide_free(e);
     mlp_free(e);
+    pn_free(e);
     for (auto& ev : e.ev) if (ev) (void)hipEventDestroy(ev);
     if (e.wev) {
         for (int k = 0; k < 3 * kWindow; k++) if (e.wev[k]) (void)hipEventDestroy(e.wev[k]);
@@ -198,6 +211,9 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     e->device = device;
     e->capacity = capacity;
     cm_default_config(&e->cfg);
+    cm_default_pn_config(&e->pn);
+    cm_default_softening_config(&e->soft);
+    cm_default_collision_config(&e->coll_cfg);
     auto fail = [&](int code) {
         cm::free_all(*e);
         delete e;
@@ -388,15 +404,63 @@ int32_t cm_verlet_step(void* h, double softening, double dt, int32_t n_steps) {
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "bad softening or step count");
     CM_HIP_TRY(*e, hipSetDevice(e->device));
     for (int s = 0; s < n_steps; s++) {
-        int rc = cm::launch_kick_drift(*e, dt);
-        if (rc == CM_OK) rc = cm::compute_forces(*e, softening);
-        if (rc == CM_OK) rc = cm::launch_kick_rotate(*e, dt);
+        int rc = cm::verlet_once(*e, softening, dt);
         if (rc != CM_OK) return rc;
     }
     int rc = cm::finalize_stats(*e);
     if (rc != CM_OK) return rc;
     if (e->stats.flags & 1)
         return cm::set_error(*e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");
+    return CM_OK;
+}
+
+int32_t cm_yoshida4_step(void* h, double softening, double dt, int32_t n_steps) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!(softening >= 0.0) || n_steps < 0)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "bad softening or step count");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    // YoshidaCoefficients (engine/include/celestial/sim/yoshida.hpp:18-28)
+    const double cbrt2 = 1.2599210498948732;
+    const double w1 = 1.0 / (2.0 - cbrt2);
+    const double w0 = -cbrt2 / (2.0 - cbrt2);
+    const double sub[3] = {w1, w0, w1};
+    for (int s = 0; s < n_steps; s++) {
+        for (int k = 0; k < 3; k++) {
+            int rc = cm::verlet_once(*e, softening, sub[k] * dt);
+            if (rc != CM_OK) return rc;
+        }
+    }
+    int rc = cm::finalize_stats(*e);
+    if (rc != CM_OK) return rc;
+    if (e->stats.flags & 1)
+        return cm::set_error(*e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");
+    return CM_OK;
+}
+
+int32_t cm_default_softening_config(cm_softening_config* cfg) {
+    if (!cfg) return CM_ERR_INVALID_ARGUMENT;
+    // SofteningConfig defaults (engine/include/celestial/sim/simulation_config.hpp:20-33)
+    cfg->mode = CM_SOFTENING_GLOBAL;
+    for (int k = 0; k < 16; k++) cfg->type_softening[k] = 0.0;
+    cfg->adaptive_scale = 0.01;
+    cfg->adaptive_min = 1e-8;
+    return CM_OK;
+}
+
+int32_t cm_set_softening(void* h, const cm_softening_config* cfg) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!cfg) return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "null softening config");
+    if (cfg->mode < CM_SOFTENING_GLOBAL || cfg->mode > CM_SOFTENING_ADAPTIVE)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "unknown softening mode");
+    for (int k = 0; k < 16; k++)
+        if (!(cfg->type_softening[k] >= 0.0))
+            return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "type softening must be >= 0");
+    if (!(cfg->adaptive_scale >= 0.0) || !(cfg->adaptive_min >= 0.0))
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT,
+                             "adaptive scale and minimum must be >= 0");
+    e->soft = *cfg;
     return CM_OK;
 }
 
@@ -464,7 +528,62 @@ int32_t cm_resolve_merges(void* h, int32_t* n_merged) {
     Engine* e = as_engine(h);
     if (!e) return CM_ERR_NOT_INITIALIZED;
     CM_HIP_TRY(*e, hipSetDevice(e->device));
-    return cm::collide_resolve(*e, n_merged);
+    cm_collision_config merge_only;
+    cm_default_collision_config(&merge_only);
+    return cm::collide_resolve(*e, merge_only, n_merged);
+}
+
+int32_t cm_default_collision_config(cm_collision_config* cfg) {
+    if (!cfg) return CM_ERR_INVALID_ARGUMENT;
+    // CollisionResolver's field defaults (Collisions/CollisionResolver.cs:28-32)
+    cfg->mode = CM_COLLISION_MERGE_ONLY;
+    cfg->default_restitution = 0.15;
+    cfg->fragmentation_specific_energy_threshold = 0.6;
+    cfg->fragmentation_mass_loss_cap = 0.3;
+    cfg->capture_velocity_factor = 0.9;
+    cfg->promote_compact_remnants = 0;
+    cfg->max_merges_per_frame = 0;
+    cfg->max_merges_per_body = 0;
+    return CM_OK;
+}
+
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+int32_t cm_configure_collisions(void* h, const cm_collision_config* cfg) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!cfg) return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "null collision config");
+    if (cfg->mode < CM_COLLISION_MERGE_ONLY || cfg->mode > CM_COLLISION_REALISTIC)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "unknown collision mode");
+    if (!(cfg->default_restitution == cfg->default_restitution) ||
+        !(cfg->fragmentation_mass_loss_cap == cfg->fragmentation_mass_loss_cap))
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "NaN collision parameter");
+    cm_collision_config c = *cfg;
+    // CollisionResolver.Configure (Collisions/CollisionResolver.cs:38-49)
+    c.default_restitution = clampd(c.default_restitution, 0.0, 1.0);
+    c.fragmentation_specific_energy_threshold =
+        std::max(1e-8, c.fragmentation_specific_energy_threshold);
+    c.fragmentation_mass_loss_cap = clampd(c.fragmentation_mass_loss_cap, 0.0, 0.9);
+    c.capture_velocity_factor = std::max(0.0, c.capture_velocity_factor);
+    if (c.fragmentation_mass_loss_cap < 0.05)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT,
+                             "fragmentation_mass_loss_cap below 0.05 (the reference's "
+                             "Math.Clamp throws on fragmentation-class events)");
+    if (c.promote_compact_remnants != 0 && c.promote_compact_remnants != 1)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "promote_compact_remnants must be 0 or 1");
+    if ((c.max_merges_per_frame > 0 || c.max_merges_per_body > 0) &&
+        c.mode != CM_COLLISION_MERGE_ONLY)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT,
+                             "merge caps apply to the MergeOnly mode only");
+    e->coll_cfg = c;
+    return CM_OK;
+}
+
+int32_t cm_resolve_collisions(void* h, int32_t* n_resolved) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    return cm::collide_resolve(*e, e->coll_cfg, n_resolved);
 }
 
 int32_t cm_compact(void* h, int32_t* new_count, int32_t* old_index) {

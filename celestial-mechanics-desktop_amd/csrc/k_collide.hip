@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "cm_internal.h"
+#include "cm_math.h"
 
 namespace cm {
 
@@ -63,6 +64,9 @@ struct Coll {
     CollHeader* hdr = nullptr;
     int32_t* scan = nullptr;       // compaction: per-block sums
     int32_t* new_index = nullptr;
+    uint8_t* caps_act = nullptr;   // merge caps: active flags as the decision pass sees them
+    int32_t* caps_count = nullptr; // merge caps: merges per body this call
+    uint8_t* caps_allow = nullptr; // merge caps: per event
     int n_events = 0;              // of the last detection
 };
 
@@ -73,7 +77,7 @@ void collide_free(Engine& e) {
     if (!c) return;
     void* ptrs[] = {c->ka, c->kb, c->va, c->vb, c->hist, c->raw, c->ev, c->eka,
                     c->ekb, c->eva, c->evb, c->first, c->done, c->hdr, c->scan,
-                    c->new_index};
+                    c->new_index, c->caps_act, c->caps_count, c->caps_allow};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     delete c;
     e.coll = nullptr;
@@ -347,67 +351,275 @@ int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne_
 }
 
 // ----------------------------------------------------------- resolution ----
-// Correctly rounded cube root by a fixed IEEE sequence (see DensityModel use).
-__device__ double cube_residual(double c, double x) {
-    double c2 = c * c;
-    double e2 = fma(c, c, -c2);
-    double p = c2 * c;
-    double ep = fma(c2, c, -p);
-    double qv = e2 * c;
-    double eq = fma(e2, c, -qv);
-    return ((p - x) + ep) + (qv + eq);
-}
-
-__device__ double cbrt_rn(double x) {
-    if (x == 0.0 || !isfinite(x)) return x;
-    bool neg = x < 0.0;
-    double a = neg ? -x : x;
-    int ex;
-    double f = frexp(a, &ex);
-    int q = ex >= 0 ? ex / 3 : -((-ex + 2) / 3);
-    int r = ex - 3 * q;
-    double t = ldexp(f, r);
-    double y = 0.5874 + t * (0.4820 + t * (-0.0901 + t * 0.0078));
-    for (int it = 0; it < 4; it++) {
-        double y2 = y * y;
-        y = y - (y2 * y - t) / (3.0 * y2);
-    }
-    y = ldexp(y, q);
-    double lo = nextafter(y, 0.0), hi = nextafter(y, INFINITY);
-    double rb = fabs(cube_residual(y, a));
-    double rl = fabs(cube_residual(lo, a));
-    double rh = fabs(cube_residual(hi, a));
-    double best = y;
-    if (rl < rb) { best = lo; rb = rl; }
-    if (rh < rb) best = hi;
-    return neg ? -best : best;
-}
-
-// DensityModel.ComputeBodyRadius (Types/DensityModel.cs:54-100).
-__device__ double body_radius(double mass, double density, int32_t type) {
-    if (type == 7) {   // BodyType.BlackHole: Schwarzschild radius in AU
-        if (mass <= 0.0) return 0.0;
-        double kg = mass * 1.989e30;
-        double rm = 2.0 * 6.67430e-11 * kg / (2.998e8 * 2.998e8);
-        return rm / 1.496e11;
-    }
-    if (mass <= 0.0 || density <= 0.0) return 0.0;
-    double volume = mass / density;
-    return cbrt_rn(3.0 * volume / (4.0 * 3.14159265358979323846));
-}
-
 struct SoaPtrs {
     double *px, *py, *pz, *vx, *vy, *vz, *ax, *ay, *az, *oax, *oay, *oaz, *m, *rad, *dens;
     uint8_t *act, *col;
     int32_t* type;
 };
 
+struct ResolveCfg {
+    int mode;              // CM_COLLISION_*
+    double restitution;    // default restitution of types without their own
+    double mass_loss_cap;  // fragmentation mass-loss cap
+    int promote;           // PromoteCompactRemnant after merges
+};
+
+// .NET Math.Max / Math.Clamp for doubles.
+__device__ __forceinline__ double cs_max(double a, double b) {
+    return (a > b || isnan(a)) ? a : b;
+}
+__device__ __forceinline__ double cs_clamp(double v, double lo, double hi) {
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+constexpr double kSolarMassKg = 1.989e30;
+constexpr double kAuM = 1.496e11;
+constexpr double kTimeUnitS = 5.0226e6;
+constexpr double kGSi = 6.67430e-11;
+
+// CollisionEnergyModel.Evaluate (Astrophysics/CollisionEnergyModel.cs:20-70):
+// the classification and the ejecta mass, the fields that change the state.
+__device__ void collision_energy(double m1, double m2, double eff_radius_au,
+                                 double rel_speed, double max_loss, bool& is_merge,
+                                 bool& is_frag, double& ejecta_kg) {
+    const double m1kg = cs_max(m1, 1e-15) * kSolarMassKg;
+    const double m2kg = cs_max(m2, 1e-15) * kSolarMassKg;
+    const double mt = m1kg + m2kg;
+    const double rm = cs_max(eff_radius_au, 1e-12) * kAuM;
+    const double v = (cs_max(rel_speed, 0.0) * kAuM) / kTimeUnitS;
+    const double mu = (m1kg * m2kg) / cs_max(mt, 1e-15);
+    const double ec = 0.5 * mu * v * v;
+    const double eb = (3.0 / 5.0) * kGSi * mt * mt / cs_max(rm, 1.0);
+    const double ratio = eb > 0.0 ? ec / eb : 0.0;
+    const bool merge = ec < 0.5 * eb;
+    const bool frag = !merge && ec < eb;
+    double f = 0.0;
+    if (frag) {
+        const double nrm = cs_clamp((ratio - 0.5) / 0.5, 0.0, 1.0);
+        f = cs_clamp(0.08 + nrm * 0.32, 0.05, max_loss);
+    } else if (!merge) {
+        const double nrm = cs_clamp(ratio - 1.0, 0.0, 12.0) / 12.0;
+        const double ccap = cs_max(max_loss, 0.55);
+        f = cs_clamp(0.5 + nrm * 0.4, 0.45, ccap);
+    }
+    is_merge = merge;
+    is_frag = frag;
+    ejecta_kg = mt * f;
+}
+
+// CollisionResolver.GetRestitutionForType (Collisions/CollisionResolver.cs:456-470);
+// BodyType: Star 0, Planet 1, GasGiant 2, RockyPlanet 3, Moon 4, Asteroid 5,
+// NeutronStar 6, BlackHole 7, Comet 8, Custom 9 (Types/BodyType.cs).
+__device__ __forceinline__ double restitution_for(int32_t t, double dflt) {
+    switch (t) {
+        case 7: return 0.0;
+        case 6: return 0.02;
+        case 0: return 0.05;
+        case 2: return 0.08;
+        case 1: case 3: return 0.2;
+        case 5: return 0.32;
+        case 8: return 0.38;
+        case 4: return 0.18;
+        default: return dflt;
+    }
+}
+
+__device__ __forceinline__ bool is_compact(int32_t t) { return t == 6 || t == 7; }
+
+// DensityModel.GetDefaultDensity (Types/DensityModel.cs:76-87).
+__device__ __forceinline__ double default_density(int32_t t) {
+    switch (t) {
+        case 0: return 1000.0;
+        case 1: case 3: return 5000.0;
+        case 2: return 1300.0;
+        case 4: return 3300.0;
+        case 5: return 3000.0;
+        case 6: return 1e12;
+        case 8: return 500.0;
+        case 7: return 1.0;
+        default: return 1000.0;
+    }
+}
+
+// ResolveImpulse (CollisionResolver.cs:378-405).
+__device__ void impulse(const SoaPtrs& s, int a, int b, double nx, double ny, double nz,
+                        double rns, double e) {
+    if (rns >= 0.0) return;
+    const double ma = s.m[a], mb = s.m[b];
+    if (ma <= 0.0 || mb <= 0.0) return;
+    const double ia = 1.0 / ma, ib = 1.0 / mb;
+    const double j = -(1.0 + e) * rns / (ia + ib);
+    s.vx[a] -= j * ia * nx; s.vy[a] -= j * ia * ny; s.vz[a] -= j * ia * nz;
+    s.vx[b] += j * ib * nx; s.vy[b] += j * ib * ny; s.vz[b] += j * ib * nz;
+}
+
+// PositionalCorrection (CollisionResolver.cs:407-437).
+__device__ void positional(const SoaPtrs& s, int a, int b, double depth, double nx,
+                           double ny, double nz) {
+    if (depth <= 0.0) return;
+    const double ma = s.m[a], mb = s.m[b];
+    if (ma <= 0.0 || mb <= 0.0) return;
+    const double ia = 1.0 / ma, ib = 1.0 / mb;
+    const double c = 0.6 * cs_max(depth - 1e-4, 0.0) / (ia + ib);
+    s.px[a] -= c * ia * nx; s.py[a] -= c * ia * ny; s.pz[a] -= c * ia * nz;
+    s.px[b] += c * ib * nx; s.py[b] += c * ib * ny; s.pz[b] += c * ib * nz;
+}
+
+// ConservePairMomentum (CollisionResolver.cs:527-546).
+__device__ void conserve_pair(const SoaPtrs& s, int a, int b, double dx, double dy,
+                              double dz) {
+    const double ma = cs_max(s.m[a], 1e-15), mb = cs_max(s.m[b], 1e-15);
+    const double cx = ma * s.vx[a] + mb * s.vx[b];
+    const double cy = ma * s.vy[a] + mb * s.vy[b];
+    const double cz = ma * s.vz[a] + mb * s.vz[b];
+    const double ex = dx - cx, ey = dy - cy, ez = dz - cz;
+    const double it = 1.0 / (ma + mb);
+    s.vx[a] += ex * it; s.vy[a] += ey * it; s.vz[a] += ez * it;
+    s.vx[b] += ex * it; s.vy[b] += ey * it; s.vz[b] += ez * it;
+}
+
+__device__ void deactivate(const SoaPtrs& s, int b) {
+    s.act[b] = 0;
+    s.col[b] = 0;
+    s.ax[b] = 0.0; s.ay[b] = 0.0; s.az[b] = 0.0;
+    s.oax[b] = 0.0; s.oay[b] = 0.0; s.oaz[b] = 0.0;
+}
+
+// MergePolicy.Merge (Collisions/MergePolicy.cs:31-80).
+__device__ void merge_policy(const SoaPtrs& s, int a, int b) {
+    const double ma = s.m[a], mb = s.m[b];
+    const double mt = ma + mb;
+    if (mt <= 0.0) return;                          // MergePolicy.cs:37
+    const double inv = 1.0 / mt;
+    s.vx[a] = (ma * s.vx[a] + mb * s.vx[b]) * inv;
+    s.vy[a] = (ma * s.vy[a] + mb * s.vy[b]) * inv;
+    s.vz[a] = (ma * s.vz[a] + mb * s.vz[b]) * inv;
+    s.px[a] = (ma * s.px[a] + mb * s.px[b]) * inv;
+    s.py[a] = (ma * s.py[a] + mb * s.py[b]) * inv;
+    s.pz[a] = (ma * s.pz[a] + mb * s.pz[b]) * inv;
+    s.m[a] = mt;
+    s.dens[a] = (ma * s.dens[a] + mb * s.dens[b]) * inv;
+    if (s.type[b] == 7) s.type[a] = 7;
+    s.rad[a] = body_radius(mt, s.dens[a], s.type[a]);
+    deactivate(s, b);
+}
+
+// PromoteCompactRemnant (CollisionResolver.cs:577-607).
+__device__ void promote(const SoaPtrs& s, int a) {
+    const int32_t cur = s.type[a];
+    if (!(cur == 0 || cur == 6 || cur == 7)) return;
+    const double mass = s.m[a];
+    int32_t tgt = cur;
+    if (mass >= 3.0) tgt = 7;
+    else if (mass >= 1.6) tgt = 6;
+    if (tgt == cur) return;
+    const double d = default_density(tgt);
+    s.type[a] = tgt;
+    s.dens[a] = d;
+    s.rad[a] = body_radius(mass, d, tgt);
+}
+
+// CollisionResolver.Resolve's body for one event (CollisionResolver.cs:72-244):
+// returns whether the event was resolved (counted in LastCollisionCount).
+__device__ bool resolve_event(const SoaPtrs& s, const CollEvent& e, const ResolveCfg& c,
+                              bool allowed) {
+    int a = e.a, b = e.b;
+    if (!s.act[a] || !s.act[b]) return false;      // :76-77
+    double ma = s.m[a], mb = s.m[b];
+    if (ma <= 0.0 || mb <= 0.0) return false;       // :79-81
+    double nx = s.px[b] - s.px[a], ny = s.py[b] - s.py[a], nz = s.pz[b] - s.pz[a];
+    const double dist = sqrt(nx * nx + ny * ny + nz * nz);
+    if (dist > 1e-12) {
+        const double inv = 1.0 / dist;
+        nx *= inv; ny *= inv; nz *= inv;
+    } else {
+        nx = 1.0; ny = 0.0; nz = 0.0;
+    }
+    const double rvx = s.vx[b] - s.vx[a], rvy = s.vy[b] - s.vy[a], rvz = s.vz[b] - s.vz[a];
+    const double rns = rvx * nx + rvy * ny + rvz * nz;
+    const double rsp = sqrt(rvx * rvx + rvy * rvy + rvz * rvz);
+    bool is_merge, is_frag;
+    double ej_kg;
+    collision_energy(ma, mb, 0.5 * cs_max(s.rad[a] + s.rad[b], 1e-12), rsp, c.mass_loss_cap,
+                     is_merge, is_frag, ej_kg);
+    // SelectOutcome (:249-267): 0 Bounce, 1 Merge, 2 Fragmentation, 3 Accretion,
+    // 4 CatastrophicDisruption; the mode overrides it (:123-126)
+    int outcome;
+    if (is_compact(s.type[a]) || is_compact(s.type[b])) outcome = 3;
+    else if (is_merge) outcome = 1;
+    else if (is_frag) outcome = 2;
+    else outcome = 4;
+    if (c.mode == CM_COLLISION_BOUNCE_ONLY) outcome = 0;
+    else if (c.mode == CM_COLLISION_MERGE_ONLY) outcome = 1;
+    if (outcome == 3 && !is_compact(s.type[a]) && is_compact(s.type[b])) {
+        const int t = a; a = b; b = t;              // EnsureAccretorSurvivor (:565-575)
+    }
+    ma = s.m[a];
+    mb = s.m[b];
+    if (outcome == 1 || outcome == 3) {
+        if (!allowed) return false;                 // the engine's merge caps
+        merge_policy(s, a, b);
+        if (c.promote) promote(s, a);
+    } else if (outcome == 2) {
+        const double r = cs_clamp(0.5 * (restitution_for(s.type[a], c.restitution) +
+                                         restitution_for(s.type[b], c.restitution)),
+                                  0.0, 1.0);
+        impulse(s, a, b, nx, ny, nz, rns, r * 0.35);
+        positional(s, a, b, e.overlap, nx, ny, nz);
+        const double ej = ej_kg / kSolarMassKg;
+        const double ef = cs_clamp(ej / cs_max(ma + mb, 1e-15), 0.02, c.mass_loss_cap);
+        const double keep = 1.0 - ef;
+        const double pbx = ma * s.vx[a] + mb * s.vx[b];
+        const double pby = ma * s.vy[a] + mb * s.vy[b];
+        const double pbz = ma * s.vz[a] + mb * s.vz[b];
+        s.m[a] = cs_max(1e-12, ma * keep);
+        s.m[b] = cs_max(1e-12, mb * keep);
+        s.rad[a] = body_radius(s.m[a], s.dens[a], s.type[a]);
+        s.rad[b] = body_radius(s.m[b], s.dens[b], s.type[b]);
+        conserve_pair(s, a, b, pbx, pby, pbz);
+    } else if (outcome == 4) {
+        // ResolveCatastrophicDisruption (:285-333)
+        const double mt = ma + mb;
+        if (mt > 0.0) {
+            const double sm = cs_max(1e-12, mt - ej_kg / kSolarMassKg);
+            const double qx = ma * s.vx[a] + mb * s.vx[b];
+            const double qy = ma * s.vy[a] + mb * s.vy[b];
+            const double qz = ma * s.vz[a] + mb * s.vz[b];
+            s.px[a] = (ma * s.px[a] + mb * s.px[b]) / mt;
+            s.py[a] = (ma * s.py[a] + mb * s.py[b]) / mt;
+            s.pz[a] = (ma * s.pz[a] + mb * s.pz[b]) / mt;
+            s.vx[a] = qx / sm; s.vy[a] = qy / sm; s.vz[a] = qz / sm;
+            s.m[a] = sm;
+            const int32_t ta = s.type[a], tb = s.type[b];
+            const int32_t st = (is_compact(ta) || is_compact(tb)) ? (is_compact(ta) ? ta : tb)
+                                                                  : (ma >= mb ? ta : tb);
+            s.type[a] = st;
+            s.rad[a] = body_radius(s.m[a], s.dens[a], st);
+            deactivate(s, b);
+        }
+    } else {
+        const double r = cs_clamp(0.5 * (restitution_for(s.type[a], c.restitution) +
+                                         restitution_for(s.type[b], c.restitution)),
+                                  0.0, 1.0);
+        impulse(s, a, b, nx, ny, nz, rns, r);
+        positional(s, a, b, e.overlap, nx, ny, nz);
+    }
+    return true;
+}
+
+// The event list in the reference's order semantics, by rounds: an event runs
+// once every earlier pending event that shares one of its bodies has run.
+// Each event reads and writes only its own two bodies, so events applied
+// together commute and the result is the sequential loop's, bit for bit.
 __global__ void __launch_bounds__(1024)
 k_resolve(int ne, const CollEvent* __restrict__ ev, uint8_t* __restrict__ done,
-          int32_t* __restrict__ first, SoaPtrs s, CollHeader* __restrict__ h) {
+          int32_t* __restrict__ first, SoaPtrs s, ResolveCfg cfg,
+          const uint8_t* __restrict__ allow, CollHeader* __restrict__ h) {
     __shared__ int s_pending;
-    __shared__ int s_merged;
-    if (threadIdx.x == 0) s_merged = 0;
+    __shared__ int s_resolved;
+    if (threadIdx.x == 0) s_resolved = 0;
     for (;;) {
         if (threadIdx.x == 0) s_pending = 0;
         __syncthreads();
@@ -425,37 +637,50 @@ k_resolve(int ne, const CollEvent* __restrict__ ev, uint8_t* __restrict__ done,
         __syncthreads();
         for (int k = threadIdx.x; k < ne; k += blockDim.x) {
             if (done[k]) continue;
-            int a = ev[k].a, b = ev[k].b;
-            if (first[a] != k || first[b] != k) { atomicAdd(&s_pending, 1); continue; }
+            const CollEvent e = ev[k];
+            if (first[e.a] != k || first[e.b] != k) { atomicAdd(&s_pending, 1); continue; }
             done[k] = 1;
-            if (!s.act[a] || !s.act[b]) continue;          // CollisionResolver.cs:76-77
-            double ma = s.m[a], mb = s.m[b];
-            if (ma <= 0.0 || mb <= 0.0) continue;           // :79-81
-            atomicAdd(&s_merged, 1);
-            double mt = ma + mb;
-            if (mt <= 0.0) continue;                        // MergePolicy.cs:37
-            double inv = 1.0 / mt;
-            s.vx[a] = (ma * s.vx[a] + mb * s.vx[b]) * inv;
-            s.vy[a] = (ma * s.vy[a] + mb * s.vy[b]) * inv;
-            s.vz[a] = (ma * s.vz[a] + mb * s.vz[b]) * inv;
-            s.px[a] = (ma * s.px[a] + mb * s.px[b]) * inv;
-            s.py[a] = (ma * s.py[a] + mb * s.py[b]) * inv;
-            s.pz[a] = (ma * s.pz[a] + mb * s.pz[b]) * inv;
-            s.m[a] = mt;
-            s.dens[a] = (ma * s.dens[a] + mb * s.dens[b]) * inv;
-            if (s.type[b] == 7) s.type[a] = 7;
-            s.rad[a] = body_radius(mt, s.dens[a], s.type[a]);
-            s.act[b] = 0;
-            s.col[b] = 0;
-            s.ax[b] = 0.0; s.ay[b] = 0.0; s.az[b] = 0.0;
-            s.oax[b] = 0.0; s.oay[b] = 0.0; s.oaz[b] = 0.0;
+            if (resolve_event(s, e, cfg, allow ? allow[k] != 0 : true))
+                atomicAdd(&s_resolved, 1);
         }
         __syncthreads();
         int pending = s_pending;
         __syncthreads();
         if (pending == 0) break;
     }
-    if (threadIdx.x == 0) h->n_merged = s_merged;
+    if (threadIdx.x == 0) h->n_merged = s_resolved;
+}
+
+// The engine's merge safeguards (engine/src/physics/collision_resolver.cpp:
+// 46-76) for MergeOnly: which events may merge, decided in list order from
+// the active flags and per-body merge counts alone (a merge keeps the
+// survivor's mass positive and deactivates the absorbed body, so no
+// arithmetic is needed).  One thread; it stops at the frame cap.
+__global__ void k_merge_caps(int ne, const CollEvent* __restrict__ ev,
+                             const double* __restrict__ m, uint8_t* __restrict__ act_copy,
+                             int32_t* __restrict__ count, uint8_t* __restrict__ allow,
+                             int max_frame, int max_body) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int total = 0;
+    for (int k = 0; k < ne; k++) {
+        const int a = ev[k].a, b = ev[k].b;
+        if (!act_copy[a] || !act_copy[b]) continue;
+        if (m[a] <= 0.0 || m[b] <= 0.0) continue;
+        if (max_frame > 0 && total >= max_frame) break;
+        if (max_body > 0 && (count[a] >= max_body || count[b] >= max_body)) continue;
+        allow[k] = 1;
+        act_copy[b] = 0;
+        count[a]++;
+        total++;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_caps_init(int n, int ne, const uint8_t* __restrict__ act, uint8_t* __restrict__ act_copy,
+            int32_t* __restrict__ count, uint8_t* __restrict__ allow) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { act_copy[i] = act[i]; count[i] = 0; }
+    if (i < ne) allow[i] = 0;
 }
 
 static SoaPtrs soa_ptrs(Engine& e) {
@@ -469,13 +694,37 @@ static SoaPtrs soa_ptrs(Engine& e) {
     return s;
 }
 
-int collide_resolve(Engine& e, int32_t* merged) {
+int collide_resolve(Engine& e, const cm_collision_config& cc, int32_t* merged) {
     Coll* c = coll(e);
     if (merged) *merged = 0;
     if (!c || c->n_events == 0) return CM_OK;
     hipStream_t st = e.stream;
-    hipLaunchKernelGGL(k_resolve, dim3(1), dim3(1024), 0, st, c->n_events, c->ev, c->done,
-                       c->first, soa_ptrs(e), c->hdr);
+    const int ne = c->n_events;
+    ResolveCfg cfg;
+    cfg.mode = cc.mode;
+    cfg.restitution = cc.default_restitution;
+    cfg.mass_loss_cap = cc.fragmentation_mass_loss_cap;
+    cfg.promote = cc.promote_compact_remnants;
+    const uint8_t* allow = nullptr;
+    if (cc.mode == CM_COLLISION_MERGE_ONLY &&
+        (cc.max_merges_per_frame > 0 || cc.max_merges_per_body > 0)) {
+        if (!c->caps_act) {
+            size_t nb = (size_t)c->cap_bodies;
+            if (hipMalloc(&c->caps_act, std::max<size_t>(nb, 256)) != hipSuccess ||
+                hipMalloc(&c->caps_count, std::max<size_t>(nb * 4, 256)) != hipSuccess ||
+                hipMalloc(&c->caps_allow, std::max<size_t>((size_t)c->cap_events, 256)) != hipSuccess)
+                return set_error(e, CM_ERR_OUT_OF_MEMORY, "merge-cap scratch allocation failed");
+        }
+        const int nmax = std::max(e.count, ne);
+        hipLaunchKernelGGL(k_caps_init, dim3((nmax + 255) / 256), dim3(256), 0, st, e.count, ne,
+                           e.act, c->caps_act, c->caps_count, c->caps_allow);
+        hipLaunchKernelGGL(k_merge_caps, dim3(1), dim3(64), 0, st, ne, c->ev, e.f[MASS],
+                           c->caps_act, c->caps_count, c->caps_allow, cc.max_merges_per_frame,
+                           cc.max_merges_per_body);
+        allow = c->caps_allow;
+    }
+    hipLaunchKernelGGL(k_resolve, dim3(1), dim3(1024), 0, st, ne, c->ev, c->done,
+                       c->first, soa_ptrs(e), cfg, allow, c->hdr);
     CM_HIP_TRY(e, hipGetLastError());
     CollHeader hh;
     CM_HIP_TRY(e, hipMemcpyAsync(&hh, c->hdr, sizeof(hh), hipMemcpyDeviceToHost, st));

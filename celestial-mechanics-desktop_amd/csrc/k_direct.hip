@@ -10,13 +10,16 @@
 //   k_direct_sweep: CpuParallelBackend (Solvers/CpuParallelBackend.cs:99-136)
 //     and the shell-theorem loop of CpuSingleThreadBackend (:124-158) with
 //     GravityKernel.AccelerationCoeffFromSource (Solvers/GravityKernel.cs:9-33).
+//   k_direct_softened: the engine's brute force with per-type or adaptive
+//     softening (engine/src/sim/engine.cpp:587-650).
 // Built with -ffp-contract=off and IEEE division / square root: results are
-// bit-identical to the C# FP64 arithmetic.
+// bit-identical to the reference's FP64 arithmetic.
 #include <hip/hip_runtime.h>
 
 #include <vector>
 
 #include "cm_internal.h"
+#include "cm_math.h"
 
 namespace cm {
 
@@ -143,6 +146,71 @@ k_direct_sweep(int n, double eps2, int shell, const double* __restrict__ px,
     if (i < n) { ax[i] = sx; ay[i] = sy; az[i] = sz; }
 }
 
+// Engine::compute_cpu_brute_forces with a SofteningMode other than Global
+// (engine/src/sim/engine.cpp:587-650): active j != i in ascending j, pairwise
+// eps = (eps_i + eps_j) * 0.5 and the engine's operation order
+// (inv3 = inv*inv*inv, factor = m_j * inv3).  eps_k is the per-type table
+// entry (type mod 16, as gravity_kernel.cu:131 masks it) or
+// max(scale * cbrt(m_k), min) with a correctly rounded cube root; each tile
+// computes its sources' eps once.
+struct SoftTable { double eps[16]; };
+
+__device__ __forceinline__ double soft_eps(int mode, const SoftTable& tb, double scale,
+                                           double mn, double mass, int32_t type) {
+    if (mode == CM_SOFTENING_PER_TYPE) return tb.eps[type & 15];
+    double e = scale * cbrt_rn(mass);
+    return e < mn ? mn : e;   // std::max(e, mn)
+}
+
+__global__ void __launch_bounds__(kTile)
+k_direct_softened(int n, int mode, SoftTable tb, double scale, double mn,
+                  const double* __restrict__ px, const double* __restrict__ py,
+                  const double* __restrict__ pz, const double* __restrict__ m,
+                  const uint8_t* __restrict__ act, const int32_t* __restrict__ type,
+                  double* __restrict__ ax, double* __restrict__ ay,
+                  double* __restrict__ az) {
+    __shared__ SrcTile t;   // t.r holds the sources' eps
+    int i = blockIdx.x * kTile + threadIdx.x;
+    bool me = i < n && act[i];
+    double xi = 0, yi = 0, zi = 0, ei = 0;
+    if (me) {
+        xi = px[i]; yi = py[i]; zi = pz[i];
+        ei = soft_eps(mode, tb, scale, mn, m[i], type[i]);
+    }
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int base = 0; base < n; base += kTile) {
+        __syncthreads();
+        int j = base + threadIdx.x;
+        if (j < n) {
+            t.x[threadIdx.x] = px[j];
+            t.y[threadIdx.x] = py[j];
+            t.z[threadIdx.x] = pz[j];
+            t.m[threadIdx.x] = m[j];
+            t.a[threadIdx.x] = act[j];
+            t.r[threadIdx.x] = act[j] ? soft_eps(mode, tb, scale, mn, m[j], type[j]) : 0.0;
+        } else {
+            t.a[threadIdx.x] = 0;
+        }
+        __syncthreads();
+        if (!me) continue;
+        int lim = min(kTile, n - base);
+        for (int q = 0; q < lim; q++) {
+            if (base + q == i || !t.a[q]) continue;
+            double e = (ei + t.r[q]) * 0.5;
+            double eps2 = e * e;
+            double dx = xi - t.x[q];
+            double dy = yi - t.y[q];
+            double dz = zi - t.z[q];
+            double d2 = dx * dx + dy * dy + dz * dz + eps2;
+            double inv = 1.0 / sqrt(d2);
+            double inv3 = inv * inv * inv;
+            double f = t.m[q] * inv3;
+            sx -= f * dx; sy -= f * dy; sz -= f * dz;
+        }
+    }
+    if (i < n) { ax[i] = sx; ay[i] = sy; az[i] = sz; }
+}
+
 // Judge sums for sampled targets: target t, source chunk c -> partial.
 // Partials over chunks of kSampleChunk sources are folded in chunk order, so
 // the result is deterministic (it differs from a single sequential sweep
@@ -208,7 +276,14 @@ int launch_direct(Engine& e, double softening) {
     e.mark(0);
     e.mark(1);
     bool pairwise = e.cfg.kind == CM_BACKEND_DIRECT_PAIRWISE && !e.cfg.shell_theorem;
-    if (pairwise) {
+    if (e.soft.mode != CM_SOFTENING_GLOBAL) {
+        SoftTable tb;
+        for (int k = 0; k < 16; k++) tb.eps[k] = e.soft.type_softening[k];
+        hipLaunchKernelGGL(k_direct_softened, dim3(nb), dim3(kTile), 0, e.stream, n,
+                           e.soft.mode, tb, e.soft.adaptive_scale, e.soft.adaptive_min,
+                           e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, e.type,
+                           e.f[AX], e.f[AY], e.f[AZ]);
+    } else if (pairwise) {
         hipLaunchKernelGGL(k_direct_pairwise, dim3(nb), dim3(kTile), 0, e.stream,
                            n, eps2, e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act,
                            e.f[AX], e.f[AY], e.f[AZ]);

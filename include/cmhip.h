@@ -185,10 +185,105 @@ int32_t cm_detect_collisions(void* engine, cm_collision_event* out,
  * (Collisions/CollisionResolver.cs:58-130, MergePolicy.cs:31-80) over the
  * events of the last cm_detect_collisions. */
 int32_t cm_resolve_merges(void* engine, int32_t* n_merged);
+/* CollisionResolver.Configure / Resolve with every outcome
+ * (Collisions/CollisionResolver.cs:36-247): MergeOnly, BounceOnly and
+ * Realistic modes (Types/CollisionMode.cs); Realistic picks Accretion
+ * (compact body survives), Merge, Fragmentation or CatastrophicDisruption
+ * from CollisionEnergyModel.Evaluate (Astrophysics/CollisionEnergyModel.cs:
+ * 20-70).  Configure's clamps apply (restitution to [0, 1], mass-loss cap to
+ * [0, 0.9]); a cap below 0.05 is rejected because the reference throws on the
+ * first fragmentation-class event with it.  The two unused Configure
+ * arguments are kept for the signature.  max_merges_per_frame /
+ * max_merges_per_body are the engine's merge safeguards
+ * (engine/include/celestial/sim/simulation_config.hpp:52-60,
+ * engine/src/physics/collision_resolver.cpp:46-76), MergeOnly only, <= 0 = no
+ * cap (the C# default).  Burst telemetry and accretion disks do not change the
+ * bodies and are not produced. */
+#define CM_COLLISION_MERGE_ONLY 0
+#define CM_COLLISION_BOUNCE_ONLY 1
+#define CM_COLLISION_REALISTIC 2
+typedef struct {
+    int32_t mode;                          /* CM_COLLISION_* */
+    double default_restitution;            /* 0.15 */
+    double fragmentation_specific_energy_threshold;  /* 0.6 (unused by Resolve) */
+    double fragmentation_mass_loss_cap;    /* 0.3 */
+    double capture_velocity_factor;        /* 0.9 (unused by Resolve) */
+    int32_t promote_compact_remnants;      /* Resolve's promoteCompactRemnants */
+    int32_t max_merges_per_frame;          /* <= 0: no cap (engine default 64) */
+    int32_t max_merges_per_body;           /* <= 0: no cap (engine default 2) */
+} cm_collision_config;
+int32_t cm_default_collision_config(cm_collision_config* cfg);
+int32_t cm_configure_collisions(void* engine, const cm_collision_config* cfg);
+/* Resolves the events of the last cm_detect_collisions with the configured
+ * mode; *n_resolved = LastCollisionCount. */
+int32_t cm_resolve_collisions(void* engine, int32_t* n_resolved);
 /* Order-preserving removal of inactive bodies, ParticleSystem::compact
  * (engine/src/physics/particle_system.cpp:156-184).  old_index (may be NULL)
  * receives, for each survivor, its index before compaction. */
 int32_t cm_compact(void* engine, int32_t* new_count, int32_t* old_index);
+
+/* ---- 1PN relativistic correction ---------------------------------------- */
+/* PostNewtonianBackend (Solvers/PostNewtonianBackend.cs:33-43), the decorator
+ * NBodySolver.SelectBackend puts around any backend when post-Newtonian
+ * corrections are on (Solvers/NBodySolver.cs:480-481): after the configured
+ * backend's Newtonian forces, every force evaluation of cm_compute_forces,
+ * cm_verlet_step and cm_yoshida4_step adds
+ * PostNewtonian1Correction.ApplyCorrections
+ * (Extensions/PostNewtonian1Correction.cs:171-188): for each active body i,
+ * the EIH term of ComputeCorrection (:66-165) summed over every j != i with
+ * m_j > 0, softened with the call's softening, in FP64 bit for bit.
+ * max_velocity_fraction_c is the engine port's field
+ * (engine/include/celestial/physics/post_newtonian.hpp:13); the C# class
+ * always uses 0.3, which is the default here. */
+typedef struct {
+    int32_t enabled;
+    double max_velocity_fraction_c;       /* 0.3 */
+    double schwarzschild_warning_factor;  /* 3.0: pairs closer than this many
+                                             Schwarzschild radii are skipped */
+} cm_pn_config;
+int32_t cm_default_pn_config(cm_pn_config* cfg);
+int32_t cm_set_post_newtonian(void* engine, const cm_pn_config* cfg);
+/* OnSchwarzschildProximityWarning (PostNewtonian1Correction.cs:47,124-129):
+ * the pairs the last force evaluation skipped as strong-field, in (i, j)
+ * order.  Up to 4096 pairs per evaluation are kept; *n_total counts all. */
+typedef struct {
+    int32_t body, other;
+    double distance, schwarzschild_radius;
+} cm_pn_warning;
+int32_t cm_pn_warnings(void* engine, cm_pn_warning* out, int32_t cap,
+                       int32_t* n_total);
+
+/* ---- softening modes of the direct sum ----------------------------------- */
+/* SofteningConfig (engine/include/celestial/sim/simulation_config.hpp:13-41)
+ * as Engine::compute_cpu_brute_forces applies it
+ * (engine/src/sim/engine.cpp:581-660): with a mode other than GLOBAL the
+ * direct backends sweep every active j != i with the pairwise softening
+ * (eps_i + eps_j) / 2, where eps_k = type_softening[body_type_k mod 16]
+ * (PER_TYPE) or max(adaptive_scale * cbrt(m_k), adaptive_min) (ADAPTIVE).
+ * GLOBAL keeps the call's softening and the C# backends' arithmetic.  As in
+ * the engine, the Barnes-Hut backend always uses the global softening.
+ * Replaces celestial_set_softening_mode / _set_type_softening /
+ * _set_adaptive_softening_scale (engine/src/interop/native_api.cpp:339-358). */
+#define CM_SOFTENING_GLOBAL 0
+#define CM_SOFTENING_PER_TYPE 1
+#define CM_SOFTENING_ADAPTIVE 2
+typedef struct {
+    int32_t mode;                 /* CM_SOFTENING_* */
+    double type_softening[16];    /* per BodyType index; engine default 0 */
+    double adaptive_scale;        /* 0.01 */
+    double adaptive_min;          /* 1e-8 */
+} cm_softening_config;
+int32_t cm_default_softening_config(cm_softening_config* cfg);
+int32_t cm_set_softening(void* engine, const cm_softening_config* cfg);
+
+/* ---- Yoshida 4th-order integrator ------------------------------------------ */
+/* n_steps x the engine's IntegratorType::Yoshida4 step
+ * (engine/src/sim/engine.cpp:1057-1081): three SoAVerletIntegrator.Step
+ * sub-steps of W1 dt, W0 dt, W1 dt with the coefficients of
+ * engine/include/celestial/sim/yoshida.hpp:18-28, forces (and the 1PN term)
+ * re-evaluated in every sub-step; device-resident. */
+int32_t cm_yoshida4_step(void* engine, double softening, double dt,
+                         int32_t n_steps);
 
 /* ---- batched ensemble of independent systems ---------------------------- */
 /* Many independent systems (each <= 2048 bodies, concatenated; system s owns

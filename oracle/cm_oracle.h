@@ -124,6 +124,31 @@ int cmo_compact(int n, double* const* f64_arrays, int n_f64,
                 uint8_t* const* u8_arrays, int n_u8, int32_t* const* i32_arrays,
                 int n_i32, const uint8_t* act, int32_t* old_index);
 
+/* ---- widening rows (cm_oracle_ext.c) ----------------------------------- */
+double cmo_c_sim(void);
+int cmo_pn_apply(int n, const double* px, const double* py, const double* pz,
+                 const double* vx, const double* vy, const double* vz,
+                 const double* m, const uint8_t* act, double eps,
+                 double max_vel_frac, double warn_factor, double* ax, double* ay,
+                 double* az, int32_t* warn_i, int32_t* warn_j, double* warn_dist,
+                 double* warn_rs, int warn_cap);
+void cmo_direct_softened(int n, const double* px, const double* py,
+                         const double* pz, const double* m, const uint8_t* act,
+                         const int32_t* type, int mode, const double* table,
+                         double scale, double mn, double global_eps, double* ax,
+                         double* ay, double* az);
+void cmo_yoshida_coefficients(double* w3);
+void cmo_collision_energy(double m1, double m2, double eff_radius_au, double rel_speed,
+                          double max_loss, int32_t* is_merge, int32_t* is_frag,
+                          double* ejecta_kg, double* ratio_out);
+int cmo_resolve_events(int ne, const cmo_event_t* ev, int n, double* px, double* py,
+                       double* pz, double* vx, double* vy, double* vz, double* ax,
+                       double* ay, double* az, double* oax, double* oay, double* oaz,
+                       double* m, double* rad, double* dens, uint8_t* act, uint8_t* col,
+                       int32_t* type, int mode, double default_restitution,
+                       double mass_loss_cap, int promote_remnants, int max_per_frame,
+                       int max_per_body, int32_t* merge_count_scratch);
+
 #ifdef __cplusplus
 }
 #endif

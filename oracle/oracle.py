@@ -115,6 +115,22 @@ def lib():
     L.cmo_body_radius.restype = D
     L.cmo_resolve_merge.argtypes = [I, C.c_void_p, I] + [f64p] * 15 + [u8p, u8p, i32p]
     L.cmo_resolve_merge.restype = I
+    L.cmo_c_sim.argtypes = []
+    L.cmo_c_sim.restype = D
+    L.cmo_pn_apply.argtypes = [I] + [f64p] * 7 + [u8p, D, D, D, f64p, f64p, f64p,
+                                                  i32p, i32p, f64p, f64p, I]
+    L.cmo_pn_apply.restype = I
+    L.cmo_direct_softened.argtypes = [I, f64p, f64p, f64p, f64p, u8p, i32p, I, f64p,
+                                      D, D, D, f64p, f64p, f64p]
+    L.cmo_direct_softened.restype = V
+    L.cmo_yoshida_coefficients.argtypes = [f64p]
+    L.cmo_yoshida_coefficients.restype = V
+    L.cmo_collision_energy.argtypes = [D, D, D, D, D, C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_int32), C.POINTER(D), C.POINTER(D)]
+    L.cmo_collision_energy.restype = V
+    L.cmo_resolve_events.argtypes = ([I, C.c_void_p, I] + [f64p] * 15 + [u8p, u8p, i32p]
+                                     + [I, D, D, I, I, I, i32p])
+    L.cmo_resolve_events.restype = I
     _lib = L
     return L
 
@@ -333,3 +349,81 @@ def compact(s):
     idx = np.nonzero(act)[0].astype(np.int32)
     out = {k: np.ascontiguousarray(v[idx]) for k, v in s.items()}
     return out, idx
+
+
+# ---------------------------------------------------------------------------
+# widening rows: 1PN decorator, softening modes, Yoshida-4 (cm_oracle_ext.c)
+# ---------------------------------------------------------------------------
+def c_sim():
+    """PhysicalConstants.C_Sim (src/CelestialMechanics.Math/PhysicalConstants.cs:37)."""
+    return lib().cmo_c_sim()
+
+
+def pn_apply(s, eps, ax, ay, az, max_vel_frac=0.3, warn_factor=3.0, warn_cap=1 << 16):
+    """PostNewtonian1Correction.ApplyCorrections on copies of (ax, ay, az);
+    returns ((ax, ay, az), warnings) with warnings a list of
+    (i, j, dist, rs) in (i, j) order."""
+    n = len(s["px"])
+    out = [np.array(a, dtype=np.float64, copy=True) for a in (ax, ay, az)]
+    wi = np.zeros(warn_cap, dtype=np.int32)
+    wj = np.zeros(warn_cap, dtype=np.int32)
+    wd = np.zeros(warn_cap)
+    wr = np.zeros(warn_cap)
+    nw = lib().cmo_pn_apply(n, *(_c(s[k], np.float64) for k in
+                                 ("px", "py", "pz", "vx", "vy", "vz", "mass")),
+                            _c(s["active"], np.uint8), eps, max_vel_frac, warn_factor,
+                            *out, wi, wj, wd, wr, warn_cap)
+    k = min(nw, warn_cap)
+    warns = [(int(wi[q]), int(wj[q]), float(wd[q]), float(wr[q])) for q in range(k)]
+    return tuple(out), warns, nw
+
+
+def direct_softened(s, mode, table=None, scale=0.01, mn=1e-8, global_eps=1e-4):
+    """Engine::compute_cpu_brute_forces with SofteningMode `mode`."""
+    n = len(s["px"])
+    t = np.zeros(16) if table is None else _c(table, np.float64)
+    ax, ay, az = (np.zeros(n) for _ in range(3))
+    lib().cmo_direct_softened(n, *(_c(s[k], np.float64) for k in ("px", "py", "pz", "mass")),
+                              _c(s["active"], np.uint8), _c(s["body_type"], np.int32),
+                              int(mode), t, scale, mn, global_eps, ax, ay, az)
+    return ax, ay, az
+
+
+def yoshida_coefficients():
+    w = np.zeros(3)
+    lib().cmo_yoshida_coefficients(w)
+    return w
+
+
+def yoshida_step(s, dt, force_fn):
+    """One Yoshida-4 step (engine/src/sim/engine.cpp:1057-1066, sub-step
+    :685-745 without collisions): three Verlet sub-steps of W1 dt, W0 dt, W1 dt."""
+    for w in yoshida_coefficients():
+        verlet_step(s, w * dt, force_fn)
+
+
+def collision_energy(m1, m2, eff_radius_au, rel_speed, max_loss):
+    """CollisionEnergyModel.Evaluate: the fields that change the state."""
+    im, fr = C.c_int32(), C.c_int32()
+    ej, ratio = C.c_double(), C.c_double()
+    lib().cmo_collision_energy(float(m1), float(m2), float(eff_radius_au), float(rel_speed),
+                               float(max_loss), C.byref(im), C.byref(fr), C.byref(ej),
+                               C.byref(ratio))
+    return {"is_merge": bool(im.value), "is_fragmentation": bool(fr.value),
+            "is_catastrophic": not im.value and not fr.value, "ejecta_kg": ej.value,
+            "ratio": ratio.value}
+
+
+def resolve_events(events, s, mode=0, restitution=0.15, mass_loss_cap=0.3, promote=False,
+                   max_per_frame=0, max_per_body=0):
+    """CollisionResolver.Resolve (every mode) in place; returns the number of
+    events resolved."""
+    ev = np.ascontiguousarray(events, dtype=EVENT_DTYPE)
+    n = len(s["px"])
+    scratch = np.zeros(max(n, 1), dtype=np.int32)
+    return lib().cmo_resolve_events(len(ev), ev.ctypes.data, n,
+                                    *(s[k] for k in F64_FIELDS),
+                                    s["active"], s["collidable"], s["body_type"],
+                                    int(mode), float(restitution), float(mass_loss_cap),
+                                    int(bool(promote)), int(max_per_frame), int(max_per_body),
+                                    scratch)

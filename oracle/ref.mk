@@ -4,6 +4,7 @@
 REF ?= /root/reference/engine
 SRC = $(REF)/src/physics/barnes_hut_solver.cpp $(REF)/src/physics/octree_builder.cpp \
       $(REF)/src/physics/octree_pool.cpp $(REF)/src/physics/particle_system.cpp \
+      $(REF)/src/physics/post_newtonian.cpp \
       $(REF)/src/job/job_system.cpp $(REF)/src/job/job_queue.cpp $(REF)/src/core/error.cpp
 CXXFLAGS = -O2 -std=c++20 -fPIC -ffp-contract=off -I$(REF)/include -w
 

@@ -47,3 +47,25 @@ extern "C" int ref_compact(int n, double* px, double* mass, double* radius, uint
     ps.free();
     return k;
 }
+
+#include <celestial/physics/post_newtonian.hpp>
+
+// PostNewtonianCorrection::apply_corrections (engine/src/physics/post_newtonian.cpp:7-23)
+// on a ParticleSystem whose accelerations start at (ax, ay, az).
+extern "C" int ref_pn_apply(int n, const double* px, const double* py, const double* pz,
+                            const double* vx, const double* vy, const double* vz,
+                            const double* m, const uint8_t* act, double eps,
+                            double* ax, double* ay, double* az) {
+    ParticleSystem ps;
+    load(ps, n, px, py, pz, m, act);
+    for (int i = 0; i < n; i++) {
+        ps.vel_x[i] = vx[i]; ps.vel_y[i] = vy[i]; ps.vel_z[i] = vz[i];
+        ps.acc_x[i] = ax[i]; ps.acc_y[i] = ay[i]; ps.acc_z[i] = az[i];
+    }
+    celestial::physics::PostNewtonianCorrection pn;
+    pn.softening_epsilon = eps;
+    pn.apply_corrections(ps);
+    for (int i = 0; i < n; i++) { ax[i] = ps.acc_x[i]; ay[i] = ps.acc_y[i]; az[i] = ps.acc_z[i]; }
+    ps.free();
+    return 0;
+}
