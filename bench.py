@@ -42,6 +42,7 @@ import numpy as np  # noqa: E402
 
 PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FLOPS_PER_INTERACTION = 20
+LEAF_CAPACITY = 8               # bodies per leaf bucket (measured best at N = 2^20, theta 0.5-0.65)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_traverse_plummer1M_traffic.md")
 
 
@@ -51,7 +52,7 @@ def log(msg):
 
 def make_backend(cmhip, n, precision, theta, params=None):
     be = cmhip.HipPhysicsBackend(n, precision=precision, theta=theta, morton_levels=10,
-                                 leaf_capacity=16, group_size=64)
+                                 leaf_capacity=LEAF_CAPACITY, group_size=64)
     if params is not None:
         be.mlp_set_params(params)
         be.configure(learned_correction=1)
@@ -527,7 +528,7 @@ def main():
         "dtype": "f32-split (f64 state)",
         "data": "synthetic",
         "config": {"workload": "plummer_bh_verlet", "n_bodies": n, "theta": args.theta,
-                   "softening": eps, "dt": dt, "morton_bits": 30, "leaf_capacity": 16,
+                   "softening": eps, "dt": dt, "morton_bits": 30, "leaf_capacity": LEAF_CAPACITY,
                    "plummer_mass": 1.0, "plummer_a_au": 1.0, "mass_fraction_cut": 0.999,
                    "seed": 42},
         "force_err_median": fast["force_err_median"],

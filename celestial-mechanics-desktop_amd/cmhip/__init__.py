@@ -58,7 +58,8 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_mlp_gradient", "cm_mlp_collect", "cm_mlp_get_pairs", "cm_mlp_train",
            "cm_default_pn_config", "cm_set_post_newtonian", "cm_pn_warnings",
            "cm_default_softening_config", "cm_set_softening", "cm_yoshida4_step",
-           "cm_default_collision_config", "cm_configure_collisions", "cm_resolve_collisions")
+           "cm_default_collision_config", "cm_configure_collisions", "cm_resolve_collisions",
+           "cm_nbody_step")
 
 COLLISION_MERGE_ONLY, COLLISION_BOUNCE_ONLY, COLLISION_REALISTIC = 0, 1, 2
 
@@ -212,6 +213,7 @@ def load_library():
         "cm_default_collision_config": ([C.POINTER(CollisionConfig)], i32),
         "cm_configure_collisions": ([vp, C.POINTER(CollisionConfig)], i32),
         "cm_resolve_collisions": ([vp, C.POINTER(i32)], i32),
+        "cm_nbody_step": ([vp, f64, f64, i32, i32, C.POINTER(i32)], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -421,6 +423,14 @@ class HipPhysicsBackend:
     def step(self, softening, dt, n_steps=1):
         self._check(self._L.cm_verlet_step(self._h, float(softening), float(dt), int(n_steps)),
                     "cm_verlet_step")
+
+    def nbody_step(self, softening, dt, n_steps=1, collisions=True):
+        """n_steps of NBodySolver.StepSoA on the device: Verlet step, then
+        detection and resolution with the configured collision mode."""
+        k = C.c_int32()
+        self._check(self._L.cm_nbody_step(self._h, float(softening), float(dt), int(n_steps),
+                                          int(bool(collisions)), C.byref(k)), "cm_nbody_step")
+        return k.value
 
     def yoshida_step(self, softening, dt, n_steps=1):
         """n_steps of the engine's Yoshida4 integrator (three Verlet sub-steps)."""

@@ -34,7 +34,7 @@ int finalize_stats(Engine& e) {
         s.body_interactions = (int64_t)h.n_body_int;
         s.n_active = h.n_active;
         s.n_nodes = h.n_nodes;
-        s.n_groups = (e.count + kWave - 1) / kWave;
+        s.n_groups = h.n_groups;
         s.flags = h.overflow ? 1 : 0;
         if (e.timing) {
             s.ms_tree = ms(0, 3);
@@ -112,6 +112,42 @@ k_kick_rotate(int n, double dt, const uint8_t* __restrict__ act,
     oax[i] = a0; oay[i] = a1; oaz[i] = a2;
 }
 
+// Phases 4+5 of one step fused with phases 1+2 of the next: the same
+// operations in the same order on each body, in one pass over the arrays
+// (v += (0.5 dt0) a; a_old = a; v += (0.5 dt1) a_old; x += dt1 v).
+__global__ void __launch_bounds__(256)
+k_kick_rotate_drift(int n, double dt0, double dt1, const uint8_t* __restrict__ act,
+                    double* __restrict__ px, double* __restrict__ py,
+                    double* __restrict__ pz, double* __restrict__ vx,
+                    double* __restrict__ vy, double* __restrict__ vz,
+                    const double* __restrict__ ax, const double* __restrict__ ay,
+                    const double* __restrict__ az, double* __restrict__ oax,
+                    double* __restrict__ oay, double* __restrict__ oaz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a0 = ax[i], a1 = ay[i], a2 = az[i];
+    oax[i] = a0; oay[i] = a1; oaz[i] = a2;
+    if (!act[i]) return;
+    const double h0 = 0.5 * dt0, h1 = 0.5 * dt1;
+    const double x = (vx[i] + h0 * a0) + h1 * a0;
+    const double y = (vy[i] + h0 * a1) + h1 * a1;
+    const double z = (vz[i] + h0 * a2) + h1 * a2;
+    vx[i] = x; vy[i] = y; vz[i] = z;
+    px[i] = px[i] + dt1 * x;
+    py[i] = py[i] + dt1 * y;
+    pz[i] = pz[i] + dt1 * z;
+}
+
+int launch_kick_rotate_drift(Engine& e, double dt0, double dt1) {
+    if (e.count == 0) return CM_OK;
+    int nb = (e.count + 255) / 256;
+    hipLaunchKernelGGL(k_kick_rotate_drift, dim3(nb), dim3(256), 0, e.stream, e.count,
+                       dt0, dt1, e.act, e.f[PX], e.f[PY], e.f[PZ], e.f[VX], e.f[VY],
+                       e.f[VZ], e.f[AX], e.f[AY], e.f[AZ], e.f[OAX], e.f[OAY], e.f[OAZ]);
+    CM_HIP_TRY(e, hipGetLastError());
+    return CM_OK;
+}
+
 int launch_kick_drift(Engine& e, double dt) {
     if (e.count == 0) return CM_OK;
     int nb = (e.count + 255) / 256;
@@ -141,11 +177,18 @@ static int compute_forces(Engine& e, double softening) {
     return launch_pn(e, softening);
 }
 
-// One SoAVerletIntegrator.Step (Integrators/SoAVerletIntegrator.cs:48-115).
-static int verlet_once(Engine& e, double softening, double dt) {
-    int rc = launch_kick_drift(e, dt);
-    if (rc == CM_OK) rc = compute_forces(e, softening);
-    if (rc == CM_OK) rc = launch_kick_rotate(e, dt);
+// A run of SoAVerletIntegrator.Step calls (Integrators/SoAVerletIntegrator.cs:
+// 48-115) with step sizes dts[0..n-1]: the closing kick of each step and the
+// opening kick-drift of the next share one pass over the bodies.
+static int verlet_run(Engine& e, double softening, const double* dts, int n) {
+    if (n <= 0) return CM_OK;
+    int rc = launch_kick_drift(e, dts[0]);
+    for (int k = 0; k < n && rc == CM_OK; k++) {
+        rc = compute_forces(e, softening);
+        if (rc != CM_OK) break;
+        rc = k + 1 < n ? launch_kick_rotate_drift(e, dts[k], dts[k + 1])
+                       : launch_kick_rotate(e, dts[k]);
+    }
     return rc;
 }
 
@@ -403,8 +446,11 @@ int32_t cm_verlet_step(void* h, double softening, double dt, int32_t n_steps) {
     if (!(softening >= 0.0) || n_steps < 0)
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "bad softening or step count");
     CM_HIP_TRY(*e, hipSetDevice(e->device));
-    for (int s = 0; s < n_steps; s++) {
-        int rc = cm::verlet_once(*e, softening, dt);
+    constexpr int kChunk = 256;   // steps per host loop iteration
+    double dts[kChunk];
+    for (int k = 0; k < kChunk; k++) dts[k] = dt;
+    for (int s = 0; s < n_steps; s += kChunk) {
+        int rc = cm::verlet_run(*e, softening, dts, std::min(kChunk, n_steps - s));
         if (rc != CM_OK) return rc;
     }
     int rc = cm::finalize_stats(*e);
@@ -424,12 +470,13 @@ int32_t cm_yoshida4_step(void* h, double softening, double dt, int32_t n_steps) 
     const double cbrt2 = 1.2599210498948732;
     const double w1 = 1.0 / (2.0 - cbrt2);
     const double w0 = -cbrt2 / (2.0 - cbrt2);
-    const double sub[3] = {w1, w0, w1};
-    for (int s = 0; s < n_steps; s++) {
-        for (int k = 0; k < 3; k++) {
-            int rc = cm::verlet_once(*e, softening, sub[k] * dt);
-            if (rc != CM_OK) return rc;
-        }
+    const double sub[3] = {w1 * dt, w0 * dt, w1 * dt};
+    constexpr int kChunk = 255;   // sub-steps per host loop iteration (85 steps)
+    double dts[kChunk];
+    for (int k = 0; k < kChunk; k++) dts[k] = sub[k % 3];
+    for (int s = 0; s < n_steps; s += kChunk / 3) {
+        int rc = cm::verlet_run(*e, softening, dts, 3 * std::min(kChunk / 3, n_steps - s));
+        if (rc != CM_OK) return rc;
     }
     int rc = cm::finalize_stats(*e);
     if (rc != CM_OK) return rc;
@@ -531,6 +578,33 @@ int32_t cm_resolve_merges(void* h, int32_t* n_merged) {
     cm_collision_config merge_only;
     cm_default_collision_config(&merge_only);
     return cm::collide_resolve(*e, merge_only, n_merged);
+}
+
+int32_t cm_nbody_step(void* h, double softening, double dt, int32_t n_steps,
+                      int32_t collisions, int32_t* n_resolved) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!(softening >= 0.0) || n_steps < 0 || (collisions != 0 && collisions != 1))
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "bad softening, step count or flag");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    int32_t total = 0;
+    for (int s = 0; s < n_steps; s++) {
+        int rc = cm::verlet_run(*e, softening, &dt, 1);
+        if (rc == CM_OK && collisions) {
+            int32_t ne = 0, k = 0;
+            rc = cm::collide_detect(*e, nullptr, 0, &ne);
+            // NBodySolver resolves once per step even with no contacts
+            if (rc == CM_OK) rc = cm::collide_resolve(*e, e->coll_cfg, &k);
+            total += k;
+        }
+        if (rc != CM_OK) return rc;
+    }
+    if (n_resolved) *n_resolved = total;
+    int rc = cm::finalize_stats(*e);
+    if (rc != CM_OK) return rc;
+    if (e->stats.flags & 1)
+        return cm::set_error(*e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");
+    return CM_OK;
 }
 
 int32_t cm_default_collision_config(cm_collision_config* cfg) {

@@ -31,6 +31,7 @@
 namespace cm {
 
 constexpr int kEnsMaxBodies = 2048;   // bodies per system held in LDS
+constexpr int kEnsMaxParts = 16;      // source ranges per target of a small system
 
 struct EnsArgs {
     const int32_t* offsets;
@@ -86,6 +87,8 @@ k_ens_run(EnsArgs a) {
     __shared__ double4 s_p[kEnsMaxBodies];
     __shared__ uint8_t s_a[kEnsMaxBodies];
     __shared__ int s_sys;
+    // FP32 partial sums of the source ranges of small systems: [part][slot][6]
+    __shared__ float s_red[PREC == CM_PREC_FP32_SPLIT ? 6 * NT : 1];
     const int t = threadIdx.x;
     const double half = 0.5 * a.dt;
     unsigned long long work = 0ull;
@@ -138,6 +141,54 @@ k_ens_run(EnsArgs a) {
                 // zero leaves each float sum unchanged)
                 const float e2 = (float)a.eps2;
                 const bool skip_self = !(a.eps2 > 0.0);
+                const int T = (nb + 1) / 2;   // target slots of two targets each
+                if (T < NT) {
+                    // fewer slots than threads: the idle threads take source
+                    // ranges too; thread t sums targets (slot, slot + T) over
+                    // range `part` of P contiguous ranges, and the P partial
+                    // sums are added in range order
+                    const int P = min(kEnsMaxParts, NT / T);
+                    const int slot = t % T, part = t / T;
+                    const int i0 = slot, i1 = slot + T;
+                    const bool has1 = i1 < nb;
+                    float ax0 = 0.0f, ay0 = 0.0f, az0 = 0.0f, ax1 = 0.0f, ay1 = 0.0f, az1 = 0.0f;
+                    if (part < P) {
+                        const double4 p0 = s_p[i0];
+                        const double4 p1 = s_p[has1 ? i1 : i0];
+                        const int j0 = part * nb / P, j1 = (part + 1) * nb / P;
+                        if (!skip_self) {
+                            for (int j = j0; j < j1; j++) {
+                                const double4 qj = s_p[j];
+                                pair_f(p0, qj, e2, ax0, ay0, az0);
+                                pair_f(p1, qj, e2, ax1, ay1, az1);
+                            }
+                        } else {
+                            for (int j = j0; j < j1; j++) {
+                                const double4 qj = s_p[j];
+                                if (j != i0) pair_f(p0, qj, e2, ax0, ay0, az0);
+                                if (j != i1) pair_f(p1, qj, e2, ax1, ay1, az1);
+                            }
+                        }
+                        float* r = &s_red[6 * (part * T + slot)];
+                        r[0] = ax0; r[1] = ay0; r[2] = az0; r[3] = ax1; r[4] = ay1; r[5] = az1;
+                    }
+                    __syncthreads();
+                    if (part == 0) {
+                        for (int q = 1; q < P; q++) {
+                            const float* r = &s_red[6 * (q * T + slot)];
+                            ax0 += r[0]; ay0 += r[1]; az0 += r[2];
+                            ax1 += r[3]; ay1 += r[4]; az1 += r[5];
+                        }
+                        const bool act0 = s_a[i0] != 0;
+                        finish(i0, act0 ? (double)ax0 : 0.0, act0 ? (double)ay0 : 0.0,
+                               act0 ? (double)az0 : 0.0);
+                        if (has1) {
+                            const bool act1 = s_a[i1] != 0;
+                            finish(i1, act1 ? (double)ax1 : 0.0, act1 ? (double)ay1 : 0.0,
+                                   act1 ? (double)az1 : 0.0);
+                        }
+                    }
+                } else
                 for (int i0 = t; i0 < nb; i0 += 2 * NT) {
                     const int i1 = i0 + NT;
                     const bool has1 = i1 < nb;

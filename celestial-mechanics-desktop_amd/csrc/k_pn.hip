@@ -211,6 +211,9 @@ int32_t cm_set_post_newtonian(void* h, const cm_pn_config* cfg) {
     CM_HIP_TRY(*e, hipSetDevice(e->device));
     e->pn = *cfg;
     if (cfg->enabled) return cm::pn_ensure(*e);
+    // no evaluation adds the correction any more: forget the last warnings
+    if (cm::PnBuf* b = cm::pn_buf(*e))
+        CM_HIP_TRY(*e, hipMemsetAsync(b->count, 0, 4, e->stream));
     return CM_OK;
 }
 

@@ -358,6 +358,26 @@ k_traverse(TravArgs a) {
         const double cyg = (lo1 + hi1) * 0.5;
         const double czg = (lo2 + hi2) * 0.5;
         const float txf = (float)(xi - cxg), tyf = (float)(yi - cyg), tzf = (float)(zi - czg);
+        // FP32 split: a small group gives each target kl lanes (a power of two,
+        // kl * nmem <= 64); lane l sums entries sub, sub + kl, ... of every
+        // list for target l % nmem, and the kl partial sums are added in sub
+        // order at the end
+        constexpr bool kSplit = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
+        int kl = 1;
+        if constexpr (kSplit) {
+            while (2 * kl * nmem <= kWave) kl *= 2;
+        }
+        const int tl = lane % nmem, sub = lane / nmem;
+        const bool worker = lane < kl * nmem;
+        float txw = txf, tyw = tyf, tzw = tzf;
+        if constexpr (kSplit) {
+            if (worker && sub > 0) {
+                const int mt = s + tl;
+                txw = (float)(sx[mt] - cxg);
+                tyw = (float)(sy[mt] - cyg);
+                tzw = (float)(sz[mt] - czg);
+            }
+        }
         double acx = 0.0, acy = 0.0, acz = 0.0;   // monopole sums
         double apx = 0.0, apy = 0.0, apz = 0.0;   // body sums (FP32: folded partials)
         // learned correction summed over this lane's cells (lane = list
@@ -492,7 +512,7 @@ k_traverse(TravArgs a) {
                 uint64_t selfm = __ballot(lane < cnt && s_part[lane] >= s && s_part[lane] < e);
                 n_body += (unsigned long long)cnt * nmem - (unsigned long long)__popcll(selfm);
             }
-            if (member) {
+            if (kSplit ? worker : member) {
                 if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
                     const float e2 = (float)a.eps2;
@@ -500,7 +520,7 @@ k_traverse(TravArgs a) {
                         // with eps2 > 0 the self term (dx = 0) is exactly zero;
                         // entries past cnt are massless and far away
                         auto term = [&](const float4 v) {
-                            float dx = txf - v.x, dy = tyf - v.y, dz = tzf - v.z;
+                            float dx = txw - v.x, dy = tyw - v.y, dz = tzw - v.z;
                             float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
                             float inv = __builtin_amdgcn_rsqf(r2);
                             float f = v.w * (inv * inv * inv);
@@ -508,21 +528,33 @@ k_traverse(TravArgs a) {
                             gy = __builtin_fmaf(-f, dy, gy);
                             gz = __builtin_fmaf(-f, dz, gz);
                         };
-                        const int cnt4 = (cnt + 3) & ~3;
-                        float4 v0 = st.v[0], v1 = st.v[1], v2 = st.v[2], v3 = st.v[3];
-                        for (int j = 0; j < cnt4; j += 4) {
-                            // next four entries are loaded while these four compute
-                            const int jn = (j + 4) & (kWave - 1);
-                            const float4 n0 = st.v[jn], n1 = st.v[jn + 1], n2 = st.v[jn + 2],
-                                         n3 = st.v[jn + 3];
-                            term(v0); term(v1); term(v2); term(v3);
-                            v0 = n0; v1 = n1; v2 = n2; v3 = n3;
+                        if (kl == 1) {
+                            const int cnt4 = (cnt + 3) & ~3;
+                            float4 v0 = st.v[0], v1 = st.v[1], v2 = st.v[2], v3 = st.v[3];
+                            for (int j = 0; j < cnt4; j += 4) {
+                                const int jn = (j + 4) & (kWave - 1);
+                                const float4 n0 = st.v[jn], n1 = st.v[jn + 1], n2 = st.v[jn + 2],
+                                             n3 = st.v[jn + 3];
+                                term(v0); term(v1); term(v2); term(v3);
+                                v0 = n0; v1 = n1; v2 = n2; v3 = n3;
+                            }
+                        } else {
+                            // entries sub, sub + kl, ...: the padding past cnt
+                            // keeps every index below 64
+                            const int trips = (cnt + kl - 1) / kl;
+                            int t = 0;
+                            for (; t + 2 <= trips; t += 2) {
+                                const float4 v0 = st.v[sub + kl * t], v1 = st.v[sub + kl * (t + 1)];
+                                term(v0); term(v1);
+                            }
+                            if (t < trips) term(st.v[sub + kl * t]);
                         }
                     } else {
-                        for (int j = 0; j < cnt; j++) {
-                            if (st.id[j] == my) continue;
+                        const int mt = s + tl;
+                        for (int j = sub; j < cnt; j += kl) {
+                            if (st.id[j] == mt) continue;
                             float4 v = st.v[j];
-                            float dx = txf - v.x, dy = tyf - v.y, dz = tzf - v.z;
+                            float dx = txw - v.x, dy = tyw - v.y, dz = tzw - v.z;
                             float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                             float f = gk_coeff_f(d2, v.w, st.r[j], e2, shell);
                             gx = __builtin_fmaf(-f, dx, gx);
@@ -652,6 +684,16 @@ k_traverse(TravArgs a) {
             }
         }
 
+        if constexpr (kSplit) {
+            // add the kl partial sums of each target in sub order
+            for (int q = 1; q < kl; q++) {
+                const int src = lane + q * nmem;
+                const double bx = __shfl(apx, src < kWave ? src : lane, kWave);
+                const double by = __shfl(apy, src < kWave ? src : lane, kWave);
+                const double bz = __shfl(apz, src < kWave ? src : lane, kWave);
+                if (member) { apx += bx; apy += by; apz += bz; }
+            }
+        }
         if (member) {
             const int o = a.order[my];
             if constexpr (PREC == CM_PREC_FP32_SPLIT) {

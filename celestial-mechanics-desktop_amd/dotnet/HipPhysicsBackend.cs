@@ -38,6 +38,43 @@ internal static unsafe class CmHip
         public int* BodyTypeIndex;
     }
 
+    [StructLayout(LayoutKind.Sequential)]
+    public struct PnConfig
+    {
+        public int Enabled;
+        public double MaxVelocityFractionC;        // 0.3
+        public double SchwarzschildWarningFactor;  // 3.0
+    }
+
+    [StructLayout(LayoutKind.Sequential)]
+    public struct PnWarning
+    {
+        public int Body, Other;
+        public double Distance, SchwarzschildRadius;
+    }
+
+    [StructLayout(LayoutKind.Sequential)]
+    public struct SofteningConfig
+    {
+        public int Mode;                          // 0 global, 1 per body type, 2 adaptive
+        public fixed double TypeSoftening[16];
+        public double AdaptiveScale;              // 0.01
+        public double AdaptiveMin;                // 1e-8
+    }
+
+    [StructLayout(LayoutKind.Sequential)]
+    public struct CollisionConfig
+    {
+        public int Mode;                          // CollisionMode: 0 MergeOnly, 1 BounceOnly, 2 Realistic
+        public double DefaultRestitution;
+        public double FragmentationSpecificEnergyThreshold;
+        public double FragmentationMassLossCap;
+        public double CaptureVelocityFactor;
+        public int PromoteCompactRemnants;
+        public int MaxMergesPerFrame;             // <= 0: no cap
+        public int MaxMergesPerBody;              // <= 0: no cap
+    }
+
     [DllImport(Lib)] public static extern int cm_create(int capacity, int device, out IntPtr engine);
     [DllImport(Lib)] public static extern int cm_destroy(IntPtr engine);
     [DllImport(Lib)] public static extern IntPtr cm_last_error(IntPtr engine);
@@ -47,6 +84,15 @@ internal static unsafe class CmHip
     [DllImport(Lib)] public static extern int cm_download_soa(IntPtr engine, ref SoAPtrs dst, int count);
     [DllImport(Lib)] public static extern int cm_compute_forces(IntPtr engine, double softening);
     [DllImport(Lib)] public static extern int cm_verlet_step(IntPtr engine, double softening, double dt, int nSteps);
+    [DllImport(Lib)] public static extern int cm_yoshida4_step(IntPtr engine, double softening, double dt, int nSteps);
+    [DllImport(Lib)] public static extern int cm_default_pn_config(ref PnConfig cfg);
+    [DllImport(Lib)] public static extern int cm_set_post_newtonian(IntPtr engine, ref PnConfig cfg);
+    [DllImport(Lib)] public static extern int cm_pn_warnings(IntPtr engine, PnWarning* outp, int cap, out int nTotal);
+    [DllImport(Lib)] public static extern int cm_default_softening_config(ref SofteningConfig cfg);
+    [DllImport(Lib)] public static extern int cm_set_softening(IntPtr engine, ref SofteningConfig cfg);
+    [DllImport(Lib)] public static extern int cm_default_collision_config(ref CollisionConfig cfg);
+    [DllImport(Lib)] public static extern int cm_configure_collisions(IntPtr engine, ref CollisionConfig cfg);
+    [DllImport(Lib)] public static extern int cm_resolve_collisions(IntPtr engine, out int nResolved);
 
     public static void Check(IntPtr engine, int rc, string what)
     {
@@ -89,6 +135,39 @@ public sealed unsafe class HipPhysicsBackend : IPhysicsComputeBackend, IGravityM
         set { _cfg.Theta = value; CmHip.Check(_engine, CmHip.cm_configure(_engine, ref _cfg), "cm_configure"); }
     }
 
+    /// <summary>
+    /// PostNewtonianBackend around this backend (Solvers/PostNewtonianBackend.cs:33-43):
+    /// every force evaluation adds PostNewtonian1Correction's term on the device.
+    /// </summary>
+    public bool EnablePostNewtonian
+    {
+        get => _pn.Enabled != 0;
+        set
+        {
+            if (_pn.MaxVelocityFractionC == 0.0) CmHip.cm_default_pn_config(ref _pn);
+            _pn.Enabled = value ? 1 : 0;
+            CmHip.Check(_engine, CmHip.cm_set_post_newtonian(_engine, ref _pn), "cm_set_post_newtonian");
+        }
+    }
+
+    /// <summary>PostNewtonian1Correction.OnSchwarzschildProximityWarning: raised after
+    /// ComputeForces for every strong-field pair the device skipped, in (i, j) order.</summary>
+    public Action<int, int, double, double>? OnSchwarzschildProximityWarning { get; set; }
+
+    private CmHip.PnConfig _pn;
+
+    private void RaiseProximityWarnings()
+    {
+        if (_pn.Enabled == 0 || OnSchwarzschildProximityWarning == null) return;
+        var buf = new CmHip.PnWarning[4096];
+        fixed (CmHip.PnWarning* w = buf)
+        {
+            CmHip.Check(_engine, CmHip.cm_pn_warnings(_engine, w, buf.Length, out int total), "cm_pn_warnings");
+            for (int k = 0; k < Math.Min(total, buf.Length); k++)
+                OnSchwarzschildProximityWarning(buf[k].Body, buf[k].Other, buf[k].Distance, buf[k].SchwarzschildRadius);
+        }
+    }
+
     public void ComputeForces(BodySoA bodies, double softening)
     {
         if (bodies.Count > _capacity) throw new InvalidOperationException("body count exceeds backend capacity");
@@ -112,6 +191,7 @@ public sealed unsafe class HipPhysicsBackend : IPhysicsComputeBackend, IGravityM
             var outp = new CmHip.SoAPtrs { AccX = ax, AccY = ay, AccZ = az };
             CmHip.Check(_engine, CmHip.cm_download_soa(_engine, ref outp, bodies.Count), "cm_download_soa");
         }
+        RaiseProximityWarnings();
     }
 
     public void Dispose()

@@ -217,6 +217,15 @@ int32_t cm_configure_collisions(void* engine, const cm_collision_config* cfg);
 /* Resolves the events of the last cm_detect_collisions with the configured
  * mode; *n_resolved = LastCollisionCount. */
 int32_t cm_resolve_collisions(void* engine, int32_t* n_resolved);
+/* n_steps x NBodySolver.StepSoA on the device-resident state
+ * (Solvers/NBodySolver.cs:383-429): one SoAVerletIntegrator.Step with the
+ * configured backend (and 1PN decorator), then, when collisions = 1,
+ * CollisionDetector.Detect and CollisionResolver.Resolve with the configured
+ * collision mode (cm_configure_collisions), as NBodySolver does when
+ * collisions are enabled.  Like the C# path it never compacts.
+ * *n_resolved (may be NULL) = resolved events summed over the steps. */
+int32_t cm_nbody_step(void* engine, double softening, double dt, int32_t n_steps,
+                      int32_t collisions, int32_t* n_resolved);
 /* Order-preserving removal of inactive bodies, ParticleSystem::compact
  * (engine/src/physics/particle_system.cpp:156-184).  old_index (may be NULL)
  * receives, for each survivor, its index before compaction. */

@@ -144,3 +144,26 @@ def test_resolve_outcomes_bit_exact(mode, promote, frame_cap, body_cap):
     assert_state_equal(out, s)
     if frame_cap:
         assert k == frame_cap
+
+
+@pytest.mark.parametrize("mode", [cmhip.COLLISION_MERGE_ONLY, cmhip.COLLISION_REALISTIC])
+def test_nbody_step_with_collisions_bit_exact(mode):
+    """cm_nbody_step = n x NBodySolver.StepSoA (Verlet with FP64 Barnes-Hut,
+    detect, resolve in the configured mode, no compaction) against the
+    oracle's sequence, bit for bit."""
+    soa = mixed_outcome_disk(8000, seed=9)
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(soa.count, precision=cmhip.PREC_FP64, theta=0.5)
+    be.configure_collisions(mode=mode, promote_compact_remnants=True)
+    be.upload(soa)
+    eps, dt = 1e-4, 1e-3
+    k = be.nbody_step(eps, dt, 3, collisions=True)
+    kr = 0
+    for _ in range(3):
+        O.verlet_step(s, dt, lambda st: O.bh_tree(st, eps, 0.5, levels=10, leaf_cap=16,
+                                                  group_size=64)[0])
+        kr += O.resolve_events(O.detect(s), s, mode=mode, promote=True)
+    assert k == kr and k > 0
+    out = cmhip.BodySoA(soa.count)
+    be.download(out)
+    assert_state_equal(out, s)
