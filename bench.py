@@ -189,19 +189,23 @@ def disk(args, cmhip):
     log-uniform 0.5-5 AU, Rayleigh e 0.02 / i 0.01, masses log-uniform 1e-9 -
     1e-6 Msun, radii from 3 g/cm^3 inflated 100x), FP64 state; a cycle is
     one Verlet step (FP64 Barnes-Hut, theta 0.5, the bit-exact path), collision
-    detection, MergeOnly resolution and compaction, as NBodySolver runs them."""
+    detection, resolution in --collision-mode (MergeOnly by default) and
+    compaction."""
     eps, dt = 1e-4, 1e-3
     n = args.n if args.n_set else 262144
     soa = cmhip.generate_disk(n, seed=42, a_min=0.5, a_max=5.0, sigma_e=0.02, sigma_i=0.01,
                               m_min=1e-9, m_max=1e-6,
                               density=cmhip.ROCK_DENSITY_INFLATED)
     be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5)
+    mode = {"merge": cmhip.COLLISION_MERGE_ONLY, "bounce": cmhip.COLLISION_BOUNCE_ONLY,
+            "realistic": cmhip.COLLISION_REALISTIC}[args.collision_mode]
+    be.configure_collisions(mode=mode)
     be.upload(soa)
 
     def cycle():
         be.step(eps, dt, 1)
         _, ne = be.detect_collisions(0)
-        m = be.resolve_merges()
+        m = be.resolve_collisions()
         be.compact()
         return ne, m
 
@@ -214,12 +218,13 @@ def disk(args, cmhip):
         events += ne
         merged += m
     wall = time.perf_counter() - t0
-    line = {"metric": "collision cycles (step + detect + MergeOnly + compact) per second",
+    line = {"metric": f"collision cycles (step + detect + {args.collision_mode} resolve + compact) per second",
             "value": args.steps / wall, "unit": "cycles/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic",
-            "config": {"workload": "disk_collisions_merge", "n_bodies": n, "theta": 0.5,
+            "config": {"workload": f"disk_collisions_{args.collision_mode}", "n_bodies": n,
+                       "theta": 0.5,
                        "softening": eps, "dt": dt, "seed": 42},
             "body_steps_per_s": n * args.steps / wall, "events": events, "merged": merged,
             "bodies_left": be.count}
@@ -435,6 +440,8 @@ def main():
     ap.add_argument("--workload", choices=("plummer", "merger", "disk", "ensemble",
                                            "extensions"),
                     default="plummer")
+    ap.add_argument("--collision-mode", choices=("merge", "bounce", "realistic"), default="merge",
+                    help="CollisionResolver mode of --workload disk")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fast-only", action="store_true",

@@ -135,17 +135,39 @@ __device__ __forceinline__ uint64_t cell_key(int64_t cx, int64_t cy, int64_t cz)
            ((uint64_t)cz & kCellMask);
 }
 
+// Largest radius and count of the active bodies: per-block partials, then
+// one block folds them and sets the hash cell (SpatialHashBroadPhase.cs:25-33).
+constexpr int kRadBlocks = 256;
+
 __global__ void __launch_bounds__(256)
-k_max_radius(int n, const double* __restrict__ r, const uint8_t* __restrict__ act,
-             CollHeader* __restrict__ h) {
+k_max_radius_part(int n, const double* __restrict__ r, const uint8_t* __restrict__ act,
+                  double* __restrict__ part_r, int32_t* __restrict__ part_c) {
     __shared__ double s[256];
     __shared__ int sc[256];
     double m = 0.0;
     int c = 0;
-    for (int i = threadIdx.x; i < n; i += 256)
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
         if (act[i]) { m = r[i] > m ? r[i] : m; c++; }
     s[threadIdx.x] = m;
     sc[threadIdx.x] = c;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            s[threadIdx.x] = s[threadIdx.x + o] > s[threadIdx.x] ? s[threadIdx.x + o] : s[threadIdx.x];
+            sc[threadIdx.x] += sc[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part_r[blockIdx.x] = s[0]; part_c[blockIdx.x] = sc[0]; }
+}
+
+__global__ void __launch_bounds__(256)
+k_max_radius(int nparts, const double* __restrict__ part_r, const int32_t* __restrict__ part_c,
+             CollHeader* __restrict__ h) {
+    __shared__ double s[256];
+    __shared__ int sc[256];
+    s[threadIdx.x] = threadIdx.x < nparts ? part_r[threadIdx.x] : 0.0;
+    sc[threadIdx.x] = threadIdx.x < nparts ? part_c[threadIdx.x] : 0;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
@@ -301,6 +323,55 @@ k_event_gather(const CollHeader* __restrict__ h, int cap,
     done[e] = 0;
 }
 
+// Up to kSmallEvents events are ordered in one workgroup: a bitonic sort in
+// LDS on the unique composite key (depth descending, then (i, j) ascending),
+// the order the two stable radix sorts below produce.
+constexpr int kSmallEvents = 2048;
+
+__global__ void __launch_bounds__(1024)
+k_event_sort_small(const CollHeader* __restrict__ h, int cap, const CollEvent* __restrict__ raw,
+                   CollEvent* __restrict__ ev, uint8_t* __restrict__ done) {
+    __shared__ uint64_t s_hi[kSmallEvents], s_lo[kSmallEvents];
+    __shared__ int32_t s_ix[kSmallEvents];
+    const int ne = min(h->n_events, cap);
+    int P = 1;
+    while (P < ne) P <<= 1;
+    for (int e = threadIdx.x; e < P; e += blockDim.x) {
+        if (e < ne) {
+            const CollEvent x = raw[e];
+            const uint32_t i = (uint32_t)min(x.a, x.b), j = (uint32_t)max(x.a, x.b);
+            s_hi[e] = ~(uint64_t)__double_as_longlong(x.overlap);   // overlap > 0
+            s_lo[e] = ((uint64_t)i << 32) | j;
+        } else {
+            s_hi[e] = ~0ull;
+            s_lo[e] = ~0ull;
+        }
+        s_ix[e] = e;
+    }
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            for (int e = threadIdx.x; e < P; e += blockDim.x) {
+                const int o = e ^ jj;
+                if (o > e) {
+                    const bool up = (e & k) == 0;
+                    const bool gt = s_hi[e] > s_hi[o] || (s_hi[e] == s_hi[o] && s_lo[e] > s_lo[o]);
+                    if (gt == up) {
+                        uint64_t t = s_hi[e]; s_hi[e] = s_hi[o]; s_hi[o] = t;
+                        t = s_lo[e]; s_lo[e] = s_lo[o]; s_lo[o] = t;
+                        const int32_t q = s_ix[e]; s_ix[e] = s_ix[o]; s_ix[o] = q;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+        ev[e] = raw[s_ix[e]];
+        done[e] = 0;
+    }
+}
+
 int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne_out) {
     int rc = ensure_coll(e);
     if (rc != CM_OK) return rc;
@@ -310,7 +381,15 @@ int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne_
     *ne_out = 0;
     c->n_events = 0;
     if (n == 0) return CM_OK;
-    hipLaunchKernelGGL(k_max_radius, dim3(1), dim3(256), 0, st, n, e.f[RAD], e.act, c->hdr);
+    {
+        const int nparts = std::min(kRadBlocks, (n + 255) / 256);
+        double* part_r = e.red;                                   // diagnostics scratch
+        int32_t* part_c = reinterpret_cast<int32_t*>(e.red + kRadBlocks);
+        hipLaunchKernelGGL(k_max_radius_part, dim3(nparts), dim3(256), 0, st, n, e.f[RAD],
+                           e.act, part_r, part_c);
+        hipLaunchKernelGGL(k_max_radius, dim3(1), dim3(256), 0, st, nparts, part_r, part_c,
+                           c->hdr);
+    }
     int gb = (n + 255) / 256;
     hipLaunchKernelGGL(k_cell_keys, dim3(gb), dim3(256), 0, st, n, e.f[PX], e.f[PY],
                        e.f[PZ], e.act, c->hdr, c->ka, c->va);
@@ -327,7 +406,15 @@ int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne_
     if (hh.n_events > c->cap_events)
         return set_error(e, CM_ERR_POOL_EXHAUSTED, "collision event capacity exceeded");
     int ne = hh.n_events;
-    if (ne > 0) {
+    if (ne > 0 && ne <= kSmallEvents) {
+        hipLaunchKernelGGL(k_event_sort_small, dim3(1), dim3(1024), 0, st, c->hdr,
+                           c->cap_events, c->raw, c->ev, c->done);
+        CM_HIP_TRY(e, hipGetLastError());
+        if (out && cap > 0)
+            CM_HIP_TRY(e, hipMemcpyAsync(out, c->ev, (size_t)std::min(cap, ne) * sizeof(CollEvent),
+                                         hipMemcpyDeviceToHost, st));
+        CM_HIP_TRY(e, hipStreamSynchronize(st));
+    } else if (ne > 0) {
         int eb = (ne + 255) / 256;
         hipLaunchKernelGGL(k_event_pair_keys, dim3(eb), dim3(256), 0, st, c->hdr,
                            c->cap_events, c->raw, c->eka, c->eva);

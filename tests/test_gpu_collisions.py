@@ -27,8 +27,11 @@ def assert_state_equal(soa, s):
         np.testing.assert_array_equal(getattr(soa, k_soa)[:n], s[k_or], err_msg=k_soa)
 
 
-def test_detect_bit_exact():
-    soa = crowded_disk(20000)
+@pytest.mark.parametrize("n", [20000, 3000])
+def test_detect_bit_exact(n):
+    """Event set and order equal the oracle's; 20,000 bodies give thousands of
+    events (radix-sorted), 3,000 give ~100 (ordered in one workgroup)."""
+    soa = crowded_disk(n)
     s = soa_to_state(soa)
     s["collidable"][::97] = 0
     soa.is_collidable[:soa.count] = s["collidable"]
@@ -37,6 +40,7 @@ def test_detect_bit_exact():
     ev, total = be.detect_collisions(1 << 16)
     ref = O.detect(s)
     assert total == len(ref) and total > 20
+    assert (total > 2048) == (n == 20000)
     np.testing.assert_array_equal(ev["body_a"], ref["a"])
     np.testing.assert_array_equal(ev["body_b"], ref["b"])
     for f_gpu, f_ref in (("overlap_depth", "overlap"), ("distance", "dist"),
