@@ -1,4 +1,5 @@
-// k_collide.hip -- collision detection, MergeOnly resolution and compaction.
+// k_collide.hip -- collision detection, resolution in every CollisionMode and
+// compaction.
 //
 // Detection (CollisionDetector.Detect, Collisions/CollisionDetector.cs:35-135):
 //   * cell size max(2 * max radius of active bodies, 1e-6) as in
@@ -11,13 +12,16 @@
 //     candidate of larger index, so the event set is exactly the i < j pairs
 //     with d^2 < (r_i + r_j)^2 -- what the spatial hash yields;
 //   * events are ordered like the reference's: (i, j) ascending, then a
-//     stable sort by overlap depth, deepest first (two radix sorts).
-// Resolution (CollisionResolver.Resolve in MergeOnly mode,
-// Collisions/CollisionResolver.cs:58-130, with MergePolicy.Merge,
-// MergePolicy.cs:31-80): events are applied in list order semantics by
-// rounds: an event runs once every earlier pending event that shares one of
-// its bodies has run, so events applied together touch disjoint bodies and
-// the result equals the sequential loop's, bit for bit.
+//     stable sort by overlap depth, deepest first (two radix sorts, or one
+//     bitonic sort in LDS for up to 2048 events).
+// Resolution (CollisionResolver.Resolve, Collisions/CollisionResolver.cs:
+// 58-247, every mode and outcome, with MergePolicy.Merge, MergePolicy.cs:
+// 31-80): events are applied in list order semantics by rounds: an event runs
+// once every earlier pending event that shares one of its bodies has run, so
+// events applied together touch disjoint bodies and the result equals the
+// sequential loop's, bit for bit.  The engine's merge caps depend on the
+// merges of all earlier events, so a one-thread pass decides first which
+// events may merge.
 // Compaction (ParticleSystem::compact, engine/src/physics/particle_system.cpp:
 // 156-184): order-preserving scatter of all 18 arrays after a scan of the
 // active flags.

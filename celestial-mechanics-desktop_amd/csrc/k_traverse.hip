@@ -22,7 +22,8 @@
 // nearest point of the box); otherwise a leaf contributes its bodies and an
 // internal node pushes its children.  Accepted nodes and leaf bodies go to two
 // 64-entry LDS interaction lists; a full list is staged once (one entry per
-// lane) and every member lane sums all its entries.
+// lane) and every member lane sums all its entries (FP32 split: a group of at
+// most 32 bodies spreads each target over several lanes, see below).
 // FP64 (CM_PREC_FP64): monopole terms use the reference's monopole arithmetic
 // (BarnesHutBackend.cs:438-445), body terms GravityKernel's
 // (GravityKernel.cs:9-33); each lane keeps the two sums apart and adds them at

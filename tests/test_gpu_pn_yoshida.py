@@ -237,3 +237,42 @@ def test_yoshida_fourth_order_on_gpu():
     rv = err(40, False) / err(80, False)
     print(f"error ratio on halving dt: Yoshida {ry:.2f}, Verlet {rv:.2f}")
     assert 12.0 < ry < 20.0 and 3.0 < rv < 5.0
+
+
+def test_widening_entry_points_edge_cases():
+    """Empty and one-body states, zero step counts and the argument checks of
+    the widening entry points (error codes, no state change)."""
+    be = cmhip.HipPhysicsBackend(16, kind=cmhip.BACKEND_DIRECT)
+    one = cmhip.BodySoA.from_arrays(pos_x=np.array([0.3]), vel_y=np.array([0.1]),
+                                    mass=np.array([2.0]), is_active=np.ones(1, np.uint8))
+    be.set_post_newtonian(True)
+    be.upload(one)
+    be.compute_forces_resident(EPS)
+    out = cmhip.BodySoA(1)
+    be.download(out)
+    assert out.acc_x[0] == 0.0 and out.acc_y[0] == 0.0
+    assert be.pn_warnings() == ([], 0)
+    be.yoshida_step(EPS, 1e-3, 0)
+    assert be.nbody_step(EPS, 1e-3, 0) == 0
+    be.set_post_newtonian(False)
+    assert be.pn_warnings() == ([], 0)
+    with pytest.raises(cmhip.CmError):
+        be.set_softening(7)
+    with pytest.raises(cmhip.CmError):
+        be.set_softening(cmhip.SOFTENING_PER_TYPE, type_softening=[-1.0] * 16)
+    with pytest.raises(cmhip.CmError):
+        be.configure_collisions(mode=cmhip.COLLISION_REALISTIC, max_merges_per_frame=4)
+    with pytest.raises(cmhip.CmError):
+        be.configure_collisions(fragmentation_mass_loss_cap=0.01)
+    with pytest.raises(cmhip.CmError):
+        be.configure_collisions(mode=5)
+    # Configure's clamps: restitution 3 -> 1 is accepted
+    be.configure_collisions(mode=cmhip.COLLISION_BOUNCE_ONLY, default_restitution=3.0)
+    empty = cmhip.BodySoA.from_arrays(pos_x=np.zeros(0), mass=np.zeros(0),
+                                      is_active=np.zeros(0, np.uint8))
+    be.upload(empty)
+    be.compute_forces_resident(EPS)
+    assert be.detect_collisions(0)[1] == 0
+    assert be.resolve_collisions() == 0
+    assert be.nbody_step(EPS, 1e-3, 3) == 0
+    be.close()
