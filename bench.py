@@ -42,6 +42,8 @@ import numpy as np  # noqa: E402
 
 PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FLOPS_PER_INTERACTION = 20
+FLOPS_PER_FAR_CELL = 76         # Taylor coefficients of one (group, cell) far-field term
+FAR_FIELD_RATIO = 0.15          # group far-field expansion ratio of the FP32 split path
 LEAF_CAPACITY = 8               # bodies per leaf bucket (measured best at N = 2^20, theta 0.5-0.65)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_traverse_plummer1M_traffic.md")
 
@@ -50,18 +52,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_backend(cmhip, n, precision, theta, params=None):
+def make_backend(cmhip, n, precision, theta, params=None, far=0.0):
     be = cmhip.HipPhysicsBackend(n, precision=precision, theta=theta, morton_levels=10,
-                                 leaf_capacity=LEAF_CAPACITY, group_size=64)
+                                 leaf_capacity=LEAF_CAPACITY, group_size=64,
+                                 far_field_ratio=far)
     if params is not None:
         be.mlp_set_params(params)
         be.configure(learned_correction=1)
     return be
 
 
-def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample, params=None):
+def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample, params=None,
+             far=0.0):
     n = soa.count
-    be = make_backend(cmhip, n, precision, theta, params)
+    be = make_backend(cmhip, n, precision, theta, params, far)
     be.upload(soa)
     be.step(eps, dt, warmup)
     be.reset_stats()
@@ -88,6 +92,8 @@ def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample, param
         "ms_force_kernel_avg": st["acc_ms_force"] / max(1, st["acc_evaluations"]),
         "ms_force_eval_avg": st["acc_ms_total"] / max(1, st["acc_evaluations"]),
         "acc_interactions": st["acc_interactions"],
+        "acc_far_cells": st["acc_far_cells"],
+        "far_cells_per_eval": st["acc_far_cells"] / max(1, st["acc_evaluations"]),
         "acc_ms_force": st["acc_ms_force"],
         "evaluations": st["acc_evaluations"],
         "n_nodes": st["n_nodes"],
@@ -97,12 +103,12 @@ def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample, param
     }
 
 
-def host_roundtrip(cmhip, soa, theta, eps, reps=5):
+def host_roundtrip(cmhip, soa, theta, eps, reps=5, far=0.0):
     """IPhysicsComputeBackend.ComputeForces as the C# host calls it every step:
     upload the host BodySoA, evaluate forces (FP32 split), download AccX/Y/Z.
     PCIe-inclusive; reported beside the device-resident value, never as it."""
     n = soa.count
-    be = make_backend(cmhip, n, cmhip.PREC_FP32_SPLIT, theta)
+    be = make_backend(cmhip, n, cmhip.PREC_FP32_SPLIT, theta, far=far)
     work = soa.copy()
     be.compute_forces(work, eps)
     t0 = time.perf_counter()
@@ -433,6 +439,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--theta", type=float, default=0.5)
+    ap.add_argument("--far-field-ratio", type=float, default=FAR_FIELD_RATIO,
+                    help="group far-field expansion ratio of the FP32 split path (0: off)")
     ap.add_argument("--theta-tuned", type=float, default=0.65,
                     help="also time the fast path at this theta (0 to skip)")
     ap.add_argument("--theta-learned", type=float, default=0.75,
@@ -482,15 +490,22 @@ def main():
                                  com_frame=True)
     sample = np.sort(np.random.default_rng(4096).choice(n, min(4096, n),
                                                          replace=False)).astype(np.int32)
-    log(f"fast path (FP32 split, theta={args.theta}) ...")
+    far = args.far_field_ratio
+    log(f"fast path (FP32 split, theta={args.theta}, far-field ratio {far}) ...")
     fast = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta, args.steps, args.warmup,
-                    eps, dt, sample)
+                    eps, dt, sample, far=far)
     log(json.dumps(fast))
     roundtrip = None
+    plain = None
     if not args.fast_only:
         log("host round trip (upload, forces, download) ...")
-        roundtrip = host_roundtrip(cmhip, soa, args.theta, eps)
+        roundtrip = host_roundtrip(cmhip, soa, args.theta, eps, far=far)
         log(json.dumps(roundtrip))
+        if far > 0.0:
+            log(f"FP32 split without the far-field expansion (theta={args.theta}) ...")
+            plain = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta, args.steps,
+                             args.warmup, eps, dt, sample)
+            log(json.dumps(plain))
     base = None
     if not args.no_fp64:
         log("plain FP64 GPU path (theta=0.5) ...")
@@ -501,7 +516,7 @@ def main():
     if args.theta_tuned:
         log(f"fast path at theta={args.theta_tuned} ...")
         tuned = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta_tuned, args.steps,
-                         args.warmup, eps, dt, sample)
+                         args.warmup, eps, dt, sample, far=far)
         log(json.dumps(tuned))
     learned = None
     if args.theta_learned:
@@ -519,8 +534,9 @@ def main():
                                       "force_err_median": same["force_err_median"],
                                       "ms_per_step": same["ms_per_step"]}
 
-    achieved = (FLOPS_PER_INTERACTION * fast["acc_interactions"]
-                / (fast["acc_ms_force"] * 1e-3) / 1e12)
+    flops = (FLOPS_PER_INTERACTION * fast["acc_interactions"]
+             + FLOPS_PER_FAR_CELL * fast["acc_far_cells"])
+    achieved = flops / (fast["acc_ms_force"] * 1e-3) / 1e12
     line = {
         "metric": "BH body-steps/s at N=1M (theta=0.5, rel force err<=1e-3) vs plain FP64 GPU path",
         "value": fast["body_steps_per_s"],
@@ -537,18 +553,24 @@ def main():
         "config": {"workload": "plummer_bh_verlet", "n_bodies": n, "theta": args.theta,
                    "softening": eps, "dt": dt, "morton_bits": 30, "leaf_capacity": LEAF_CAPACITY,
                    "plummer_mass": 1.0, "plummer_a_au": 1.0, "mass_fraction_cut": 0.999,
-                   "seed": 42},
+                   "far_field_ratio": far, "seed": 42},
         "force_err_median": fast["force_err_median"],
         "force_err_max": fast["force_err_max"],
         "interactions_per_body": fast["interactions_per_body"],
+        "far_cells_per_eval": fast["far_cells_per_eval"],
         "cell_fraction": fast["cell_fraction"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_VALU_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU_TFLOPS,
                      "traffic": traffic_from_profile(), "kernel": "k_traverse<FP32_SPLIT>",
                      "ms_kernel_avg": fast["ms_force_kernel_avg"],
-                     "flops_per_launch": (FLOPS_PER_INTERACTION * fast["acc_interactions"]
-                                          / max(1, fast["evaluations"]))},
+                     "flops_per_launch": flops / max(1, fast["evaluations"])},
     }
+    if plain:
+        line["without_far_field"] = {"value": plain["body_steps_per_s"],
+                                     "ms_per_step": plain["ms_per_step"],
+                                     "force_err_median": plain["force_err_median"],
+                                     "interactions_per_body": plain["interactions_per_body"],
+                                     "ms_force_kernel_avg": plain["ms_force_kernel_avg"]}
     if roundtrip:
         line["host_roundtrip_pcie"] = roundtrip
     if base:

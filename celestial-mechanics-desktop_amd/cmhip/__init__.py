@@ -83,7 +83,8 @@ class BackendConfig(C.Structure):
     _fields_ = [("kind", C.c_int32), ("precision", C.c_int32),
                 ("shell_theorem", C.c_int32), ("morton_levels", C.c_int32),
                 ("theta", C.c_double), ("leaf_capacity", C.c_int32),
-                ("group_size", C.c_int32), ("learned_correction", C.c_int32)]
+                ("group_size", C.c_int32), ("learned_correction", C.c_int32),
+                ("far_field_ratio", C.c_double)]
 
 
 class PnConfig(C.Structure):
@@ -123,7 +124,8 @@ class ForceStats(C.Structure):
                 ("ms_tree", C.c_float), ("ms_sort", C.c_float),
                 ("ms_force", C.c_float), ("ms_total", C.c_float),
                 ("acc_interactions", C.c_int64), ("acc_ms_force", C.c_double),
-                ("acc_ms_total", C.c_double), ("acc_evaluations", C.c_int32)]
+                ("acc_ms_total", C.c_double), ("acc_evaluations", C.c_int32),
+                ("far_cells", C.c_int64), ("acc_far_cells", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -337,7 +339,7 @@ class HipPhysicsBackend:
     def __init__(self, capacity, kind=BACKEND_BARNES_HUT, precision=PREC_FP64,
                  theta=0.5, morton_levels=10, leaf_capacity=16,
                  group_size=64, enable_shell_theorem=False, learned_correction=False,
-                 device=0):
+                 far_field_ratio=0.0, device=0):
         L = load_library()
         h = C.c_void_p()
         rc = L.cm_create(int(capacity), int(device), C.byref(h))
@@ -356,6 +358,7 @@ class HipPhysicsBackend:
         self.config.group_size = group_size
         self.config.shell_theorem = int(enable_shell_theorem)
         self.config.learned_correction = int(learned_correction)
+        self.config.far_field_ratio = float(far_field_ratio)
         self._apply()
 
     # -- configuration ----------------------------------------------------

@@ -40,6 +40,8 @@ struct TreeHeader {
     unsigned long long n_cell_int;
     unsigned long long n_body_int;
     unsigned long long acc_int;   // cell + body terms since the last stats reset
+    unsigned long long n_far_int; // (group, cell) far-field expansion terms
+    unsigned long long acc_far;   // the same since the last stats reset
     int32_t n_groups;             // target groups formed for the traversal
     uint32_t work_next[8];        // per-XCD queues: next group of each eighth
 };

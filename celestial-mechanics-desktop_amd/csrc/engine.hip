@@ -32,6 +32,7 @@ int finalize_stats(Engine& e) {
         const TreeHeader& h = *e.hdr_host;
         s.cell_interactions = (int64_t)h.n_cell_int;
         s.body_interactions = (int64_t)h.n_body_int;
+        s.far_cells = (int64_t)h.n_far_int;
         s.n_active = h.n_active;
         s.n_nodes = h.n_nodes;
         s.n_groups = h.n_groups;
@@ -61,6 +62,7 @@ int finalize_stats(Engine& e) {
     }
     e.wn = 0;
     s.acc_interactions = (int64_t)e.hdr_host->acc_int;
+    s.acc_far_cells = (int64_t)e.hdr_host->acc_far;
     s.acc_ms_force = e.acc_ms_force;
     s.acc_ms_total = e.acc_ms_total;
     s.acc_evaluations = e.acc_evals;
@@ -68,7 +70,10 @@ int finalize_stats(Engine& e) {
     return CM_OK;
 }
 
-__global__ void k_reset_acc(TreeHeader* hdr) { hdr->acc_int = 0ull; }
+__global__ void k_reset_acc(TreeHeader* hdr) {
+    hdr->acc_int = 0ull;
+    hdr->acc_far = 0ull;
+}
 
 // ---- Verlet kernels --------------------------------------------------------
 // SoAVerletIntegrator.Step phases 1+2 (Integrators/SoAVerletIntegrator.cs:74-90):
@@ -239,6 +244,7 @@ int32_t cm_default_config(cm_backend_config* cfg) {
     cfg->leaf_capacity = 16;
     cfg->group_size = 64;
     cfg->learned_correction = 0;
+    cfg->far_field_ratio = 0.0;
     return CM_OK;
 }
 
@@ -354,6 +360,8 @@ int32_t cm_configure(void* h, const cm_backend_config* cfg) {
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "theta must be finite, >= 0");
     if (cfg->learned_correction != 0 && cfg->learned_correction != 1)
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "learned_correction must be 0 or 1");
+    if (!(cfg->far_field_ratio >= 0.0) || cfg->far_field_ratio > 0.5)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "far_field_ratio must be in [0, 0.5]");
     e->cfg = *cfg;
     return CM_OK;
 }
@@ -551,10 +559,12 @@ int32_t cm_reset_force_stats(void* h) {
     CM_HIP_TRY(*e, hipGetLastError());
     CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
     e->hdr_host->acc_int = 0ull;
+    e->hdr_host->acc_far = 0ull;
     e->acc_ms_force = 0.0;
     e->acc_ms_total = 0.0;
     e->acc_evals = 0;
     e->stats.acc_interactions = 0;
+    e->stats.acc_far_cells = 0;
     e->stats.acc_ms_force = 0.0;
     e->stats.acc_ms_total = 0.0;
     e->stats.acc_evaluations = 0;

@@ -221,7 +221,9 @@ struct TravArgs {
     double theta2;
     int levels;
     int shell;
-    unsigned long long* counters;   // [0] cell, [1] body, [2] accumulated terms
+    unsigned long long* counters;   // [0] cell, [1] body, [2] accumulated terms,
+                                    // [3] far cells, [4] accumulated far cells
+    double far_ratio2;              // kappa^2 of the far-field expansion (0: off)
     // learned far-field correction (k_traverse<FP32_SPLIT, true> only)
     const uint16_t* mlp_w;          // bf16 weights [out][in] per layer
     const float* mlp_p;             // FP32 parameters (biases)
@@ -297,11 +299,30 @@ struct Staging { double4 v[kWave]; double r[kWave]; int id[kWave]; };
 template <>
 struct Staging<CM_PREC_FP32_SPLIT> { float4 v[kWave]; float r[kWave]; int id[kWave]; };
 
+#ifndef CM_SPLIT_STACK
+#define CM_SPLIT_STACK 1536
+#endif
+#ifndef CM_SPLIT_WAVES
+#define CM_SPLIT_WAVES 4
+#endif
+// Occupancy of the traversal: the FP32 split kernel keeps a smaller LDS stack
+// and a register budget for more wavefronts per SIMD; the FP64 kernels keep
+// the stack the CPU restatement of this traversal models (kStackCap).
 template <int PREC, bool LEARNED>
-__global__ void __launch_bounds__(kWave)
+struct TravShape {
+    static constexpr bool split = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
+    static constexpr int stack = split ? CM_SPLIT_STACK : kStackCap;
+    static constexpr int waves = split ? CM_SPLIT_WAVES : 1;   // minimum per SIMD
+};
+
+template <int PREC, bool LEARNED>
+__global__ void __launch_bounds__(kWave, (TravShape<PREC, LEARNED>::waves))
 k_traverse(TravArgs a) {
-    __shared__ int s_stack[kStackCap];
+    constexpr int kCap = TravShape<PREC, LEARNED>::stack;
+    __shared__ int s_stack[kCap];
     __shared__ int s_cell[2 * kWave];
+    constexpr bool kFar = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
+    __shared__ int s_far[kFar ? 2 * kWave : 1];
     __shared__ int s_part[kWave];
     __shared__ int s_lpref[kWave];
     __shared__ int s_lstart[kWave];
@@ -323,7 +344,7 @@ k_traverse(TravArgs a) {
     const bool shell = a.shell != 0;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     const int reserve = 8 * (a.levels + 2);
-    unsigned long long n_cell = 0ull, n_body = 0ull;
+    unsigned long long n_cell = 0ull, n_body = 0ull, n_far = 0ull;
 
     // groups are in body (Morton) order; eighth q of them is queue q, served
     // first by the wavefronts of XCD q so that each XCD's L2 holds one compact
@@ -336,7 +357,10 @@ k_traverse(TravArgs a) {
         if (lane == 0)
             s_g = (int)((long long)q * ngroups / kXcds) + (int)atomicAdd(&a.hdr->work_next[q], 1u);
         __syncthreads();
-        const int g = s_g;
+        // group index, counts and list lengths are wave-uniform: keeping them
+        // in scalar registers keeps the inner loops' control and LDS
+        // addresses off the vector ALU
+        const int g = __builtin_amdgcn_readfirstlane(s_g);
         __syncthreads();
         if (g >= qhi) {
             if (++qi == kXcds) break;
@@ -359,17 +383,44 @@ k_traverse(TravArgs a) {
         const double cyg = (lo1 + hi1) * 0.5;
         const double czg = (lo2 + hi2) * 0.5;
         const float txf = (float)(xi - cxg), tyf = (float)(yi - cyg), tzf = (float)(zi - czg);
+        // far field (FP32 split): an accepted cell whose centre of mass r
+        // lies beyond R / kappa of the box centre c (R the box half
+        // diagonal) is summed for the whole group as the second-order Taylor
+        // expansion about c of its softened monopole field
+        //   a(c + d) = T0 + T1 d + (1/2) T2 d d,
+        //   T0 = M r s^-3/2,  T1_ab = 3 M r_a r_b s^-5/2 - d_ab M s^-3/2,
+        //   T2_abc = 15 M r_a r_b r_c s^-7/2
+        //            - 3 M (d_ab r_c + d_ac r_b + d_bc r_a) s^-5/2,
+        // s = |r|^2 + eps^2.  Each lane sums the coefficients of its share of
+        // the far cells; the wave adds them up once per group.
+        double far_lim = 1.7976931348623157e308;
+        if constexpr (kFar) {
+            if (a.far_ratio2 > 0.0) {
+                const double ex = hi0 - lo0, ey = hi1 - lo1, ez = hi2 - lo2;
+                far_lim = 0.25 * (ex * ex + ey * ey + ez * ez) / a.far_ratio2;
+            }
+        }
+        float f0x = 0.0f, f0y = 0.0f, f0z = 0.0f;                       // T0
+        float f1xx = 0.0f, f1yy = 0.0f, f1zz = 0.0f, f1xy = 0.0f, f1xz = 0.0f, f1yz = 0.0f;
+        float f2xxx = 0.0f, f2yyy = 0.0f, f2zzz = 0.0f, f2xxy = 0.0f, f2xxz = 0.0f;
+        float f2yyx = 0.0f, f2yyz = 0.0f, f2zzx = 0.0f, f2zzy = 0.0f, f2xyz = 0.0f;
+        int any_far = 0;
         // FP32 split: a small group gives each target kl lanes (a power of two,
         // kl * nmem <= 64); lane l sums entries sub, sub + kl, ... of every
         // list for target l % nmem, and the kl partial sums are added in sub
-        // order at the end
+        // order at the end.  The staged list is stored so that each sub's
+        // entries are contiguous: entry j sits in slot (j % kl) * W + j / kl,
+        // W = 64 / kl, so lane sub reads slots sub * W + t, t = 0, 1, ...
         constexpr bool kSplit = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
-        int kl = 1;
+        int kl = 1, klog = 0;
         if constexpr (kSplit) {
-            while (2 * kl * nmem <= kWave) kl *= 2;
+            while (2 * kl * nmem <= kWave) { kl *= 2; klog++; }
         }
         const int tl = lane % nmem, sub = lane / nmem;
         const bool worker = lane < kl * nmem;
+        const int kw = kWave >> klog;                                   // W
+        const int slot = ((lane & (kl - 1)) << (6 - klog)) | (lane >> klog);
+        const int rbase = worker ? sub * kw : 0;
         float txw = txf, tyw = tyf, tzw = tzf;
         if constexpr (kSplit) {
             if (worker && sub > 0) {
@@ -481,14 +532,15 @@ k_traverse(TravArgs a) {
 
         auto flush = [&](int cnt, bool cells) {
             if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                // pad to a multiple of four with massless far entries
-                if (lane >= cnt) st.v[lane] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+                // the slots past cnt hold massless far entries, whose terms
+                // are exactly zero, so the loops below may run past cnt
+                if (lane >= cnt) st.v[slot] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
             }
             if (lane < cnt) {
                 if (cells) {
                     double4 v = a.cm[s_cell[lane]];
                     if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                        st.v[lane] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
+                        st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
                                                  (float)(v.z - czg), (float)v.w);
                     } else {
                         st.v[lane] = v;
@@ -496,7 +548,7 @@ k_traverse(TravArgs a) {
                 } else {
                     int q = s_part[lane];
                     if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                        st.v[lane] = make_float4((float)(sx[q] - cxg), (float)(sy[q] - cyg),
+                        st.v[slot] = make_float4((float)(sx[q] - cxg), (float)(sy[q] - cyg),
                                                  (float)(sz[q] - czg), (float)sm[q]);
                         st.r[lane] = (float)sr[q];
                     } else {
@@ -529,32 +581,25 @@ k_traverse(TravArgs a) {
                             gy = __builtin_fmaf(-f, dy, gy);
                             gz = __builtin_fmaf(-f, dz, gz);
                         };
-                        if (kl == 1) {
-                            const int cnt4 = (cnt + 3) & ~3;
-                            float4 v0 = st.v[0], v1 = st.v[1], v2 = st.v[2], v3 = st.v[3];
-                            for (int j = 0; j < cnt4; j += 4) {
-                                const int jn = (j + 4) & (kWave - 1);
-                                const float4 n0 = st.v[jn], n1 = st.v[jn + 1], n2 = st.v[jn + 2],
-                                             n3 = st.v[jn + 3];
+                        // entries sub, sub + kl, ... below cnt: trips of them
+                        // for sub = 0, fewer (the rest padding) for the others
+                        const int trips = (cnt + kl - 1) >> klog;
+                        const float4* sv = &st.v[rbase];
+                        if (kw >= 4) {
+                            const int t4 = min(kw, (trips + 3) & ~3);
+                            for (int t = 0; t < t4; t += 4) {
+                                const float4 v0 = sv[t], v1 = sv[t + 1], v2 = sv[t + 2],
+                                             v3 = sv[t + 3];
                                 term(v0); term(v1); term(v2); term(v3);
-                                v0 = n0; v1 = n1; v2 = n2; v3 = n3;
                             }
                         } else {
-                            // entries sub, sub + kl, ...: the padding past cnt
-                            // keeps every index below 64
-                            const int trips = (cnt + kl - 1) / kl;
-                            int t = 0;
-                            for (; t + 2 <= trips; t += 2) {
-                                const float4 v0 = st.v[sub + kl * t], v1 = st.v[sub + kl * (t + 1)];
-                                term(v0); term(v1);
-                            }
-                            if (t < trips) term(st.v[sub + kl * t]);
+                            for (int t = 0; t < trips; t++) term(sv[t]);
                         }
                     } else {
                         const int mt = s + tl;
-                        for (int j = sub; j < cnt; j += kl) {
+                        for (int t = 0, j = sub; j < cnt; t++, j += kl) {
                             if (st.id[j] == mt) continue;
-                            float4 v = st.v[j];
+                            float4 v = st.v[rbase + t];
                             float dx = txw - v.x, dy = tyw - v.y, dz = tzw - v.z;
                             float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
                             float f = gk_coeff_f(d2, v.w, st.r[j], e2, shell);
@@ -604,11 +649,53 @@ k_traverse(TravArgs a) {
             __syncthreads();
         };
 
-        int S = 1, ncell = 0, npart = 0;
+        // one far cell per lane: add its expansion coefficients
+        auto far_flush = [&](int cnt) {
+          if constexpr (kFar) {
+            if (lane < cnt) {
+                const double4 v = a.cm[s_far[lane]];
+                const float rx = (float)(v.x - cxg), ry = (float)(v.y - cyg),
+                            rz = (float)(v.z - czg), M = (float)v.w;
+                const float s2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry,
+                                 __builtin_fmaf(rz, rz, (float)a.eps2)));
+                const float ir = __builtin_amdgcn_rsqf(s2), ir2 = ir * ir;
+                const float i3 = M * (ir * ir2), i5 = i3 * ir2, i7 = i5 * ir2;
+                f0x = __builtin_fmaf(i3, rx, f0x);
+                f0y = __builtin_fmaf(i3, ry, f0y);
+                f0z = __builtin_fmaf(i3, rz, f0z);
+                const float u = 3.0f * i5;
+                const float ux = u * rx, uy = u * ry, uz = u * rz;
+                f1xx += __builtin_fmaf(ux, rx, -i3);
+                f1yy += __builtin_fmaf(uy, ry, -i3);
+                f1zz += __builtin_fmaf(uz, rz, -i3);
+                f1xy = __builtin_fmaf(ux, ry, f1xy);
+                f1xz = __builtin_fmaf(ux, rz, f1xz);
+                f1yz = __builtin_fmaf(uy, rz, f1yz);
+                const float w = 15.0f * i7;
+                const float qx = w * rx * rx - u, qy = w * ry * ry - u, qz = w * rz * rz - u;
+                const float u2 = 2.0f * u;
+                f2xxx = __builtin_fmaf(qx - u2, rx, f2xxx);   // 15 M x^3 s^-7/2 - 9 M x s^-5/2
+                f2yyy = __builtin_fmaf(qy - u2, ry, f2yyy);
+                f2zzz = __builtin_fmaf(qz - u2, rz, f2zzz);
+                f2xxy = __builtin_fmaf(qx, ry, f2xxy);         // 15 M x^2 y s^-7/2 - 3 M y s^-5/2
+                f2xxz = __builtin_fmaf(qx, rz, f2xxz);
+                f2yyx = __builtin_fmaf(qy, rx, f2yyx);
+                f2yyz = __builtin_fmaf(qy, rz, f2yyz);
+                f2zzx = __builtin_fmaf(qz, rx, f2zzx);
+                f2zzy = __builtin_fmaf(qz, ry, f2zzy);
+                f2xyz = __builtin_fmaf(w * rx, ry * rz, f2xyz);
+            }
+            n_far += (unsigned long long)cnt;
+            any_far = 1;
+            __syncthreads();
+          }
+        };
+
+        int S = 1, ncell = 0, npart = 0, nfar = 0;
         if (lane == 0) s_stack[0] = 0;
         __syncthreads();
         while (S > 0) {
-            int lim = (kStackCap - reserve - S) / 7;
+            int lim = (kCap - reserve - S) / 7;
             if (lim < 1) lim = 1;
             int take = S < kWave ? S : kWave;
             if (take > lim) take = lim;
@@ -616,12 +703,21 @@ k_traverse(TravArgs a) {
             const int node = lane < take ? s_stack[base + lane] : -1;
             S = base;
             __syncthreads();
-            int kind = 0;   // 1 cell, 2 leaf, 3 open
+            int kind = 0;   // 1 cell, 2 leaf, 3 open, 4 far cell
             int4 nd = make_int4(-1, 0, 0, 0);
             if (node >= 0) {
                 nd = a.info[node];
-                kind = mac_kind(a.cm[node], nd, lo0, lo1, lo2, hi0, hi1, hi2, h, a.theta2,
-                                shell);
+                const double4 c = a.cm[node];
+                kind = mac_kind(c, nd, lo0, lo1, lo2, hi0, hi1, hi2, h, a.theta2, shell);
+                if constexpr (kFar) {
+                    const double rx = c.x - cxg, ry = c.y - cyg, rz = c.z - czg;
+                    if (kind == 1 && rx * rx + ry * ry + rz * rz > far_lim) kind = 4;
+                }
+            }
+            if constexpr (kFar) {
+                uint64_t fm = __ballot(kind == 4);
+                if (kind == 4) s_far[nfar + __popcll(fm & lt)] = node;
+                nfar += __popcll(fm);
             }
             // accepted cells, in lane order
             uint64_t cm_ = __ballot(kind == 1);
@@ -630,17 +726,28 @@ k_traverse(TravArgs a) {
             // opened internal nodes push their children, in lane order
             int nop = kind == 3 ? (nd.y & 255) : 0;
             int oin = wscan_incl(nop);
-            int otot = __shfl(oin, kWave - 1, kWave);
+            int otot = __builtin_amdgcn_readfirstlane(__shfl(oin, kWave - 1, kWave));
             if (kind == 3)
                 for (int c = 0; c < nop; c++) s_stack[S + oin - nop + c] = nd.x + c;
             S += otot;
             // opened leaves: body ranges, expanded below in lane order
             int nl = kind == 2 ? nd.w : 0;
             int lin = wscan_incl(nl);
-            int ltot = __shfl(lin, kWave - 1, kWave);
+            int ltot = __builtin_amdgcn_readfirstlane(__shfl(lin, kWave - 1, kWave));
             s_lpref[lane] = lin - nl;
             s_lstart[lane] = nd.z;
             __syncthreads();
+            if constexpr (kFar) {
+                if (nfar >= kWave) {
+                    far_flush(kWave);
+                    int rest = nfar - kWave;
+                    int v = lane < rest ? s_far[kWave + lane] : 0;
+                    __syncthreads();
+                    if (lane < rest) s_far[lane] = v;
+                    nfar = rest;
+                    __syncthreads();
+                }
+            }
             while (ncell >= kWave) {
                 flush(kWave, true);
                 int rest = ncell - kWave;
@@ -671,6 +778,9 @@ k_traverse(TravArgs a) {
         }
         if (ncell > 0) flush(ncell, true);
         if (npart > 0) flush(npart, false);
+        if constexpr (kFar) {
+            if (nfar > 0) far_flush(nfar);
+        }
 
         if constexpr (LEARNED) {
             // group totals of residual and gradient, then a = res + J (x - centre)
@@ -695,6 +805,32 @@ k_traverse(TravArgs a) {
                 if (member) { apx += bx; apy += by; apz += bz; }
             }
         }
+        if constexpr (kFar) {
+            if (any_far) {
+                // group sums of the coefficients (in double), then each
+                // member evaluates the expansion at its offset d from c
+                double t[19] = {f0x, f0y, f0z, f1xx, f1yy, f1zz, f1xy, f1xz, f1yz,
+                                f2xxx, f2yyy, f2zzz, f2xxy, f2xxz, f2yyx, f2yyz, f2zzx,
+                                f2zzy, f2xyz};
+                for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+                    for (int k = 0; k < 19; k++) t[k] += __shfl_xor(t[k], o, kWave);
+                if (member) {
+                    const double dx = txf, dy = tyf, dz = tzf;
+                    const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
+                    const double xy = dx * dy, xz = dx * dz, yz = dy * dz;
+                    apx += t[0] + (t[3] * dx + t[6] * dy + t[7] * dz)
+                           + 0.5 * (t[9] * xx + t[14] * yy + t[16] * zz)
+                           + (t[12] * xy + t[13] * xz + t[18] * yz);
+                    apy += t[1] + (t[6] * dx + t[4] * dy + t[8] * dz)
+                           + 0.5 * (t[12] * xx + t[10] * yy + t[17] * zz)
+                           + (t[14] * xy + t[18] * xz + t[15] * yz);
+                    apz += t[2] + (t[7] * dx + t[8] * dy + t[5] * dz)
+                           + 0.5 * (t[13] * xx + t[15] * yy + t[11] * zz)
+                           + (t[18] * xy + t[16] * xz + t[17] * yz);
+                }
+            }
+        }
         if (member) {
             const int o = a.order[my];
             if constexpr (PREC == CM_PREC_FP32_SPLIT) {
@@ -708,6 +844,10 @@ k_traverse(TravArgs a) {
         atomicAdd(&a.counters[0], n_cell);
         atomicAdd(&a.counters[1], n_body);
         atomicAdd(&a.counters[2], n_cell + n_body);
+        if (n_far) {
+            atomicAdd(&a.counters[3], n_far);
+            atomicAdd(&a.counters[4], n_far);
+        }
     }
 }
 
@@ -774,11 +914,15 @@ int launch_barnes_hut(Engine& e, double softening) {
     a.levels = e.cfg.morton_levels;
     a.shell = e.cfg.shell_theorem;
     a.counters = &e.hdr->n_cell_int;
+    a.far_ratio2 = e.cfg.far_field_ratio * e.cfg.far_field_ratio;
     a.mlp_w = learned ? mlp_weights(e) : nullptr;
     a.mlp_p = learned ? mlp_params(e) : nullptr;
     a.quad = e.nd_quad;
-    // persistent wavefronts: about 16 per CU, never more than bodies / 8
-    int nblocks = std::max(1, std::min(4096, (n + 7) / 8));
+    // persistent wavefronts: as many as fit on the 256 CUs (4 SIMDs each),
+    // never more than bodies / 8
+    const bool split = e.cfg.precision == CM_PREC_FP32_SPLIT && !learned;
+    const int per_simd = split ? TravShape<CM_PREC_FP32_SPLIT, false>::waves : 4;
+    int nblocks = std::max(1, std::min(1024 * per_simd, (n + 7) / 8));
     switch (e.cfg.precision) {
         case CM_PREC_FP64:
             hipLaunchKernelGGL((k_traverse<CM_PREC_FP64, false>), dim3(nblocks), dim3(kWave), 0, st, a);

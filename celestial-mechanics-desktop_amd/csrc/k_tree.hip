@@ -121,6 +121,7 @@ k_bbox_final(int nblocks, const double* __restrict__ part,
         hdr->level_start[1] = 1;
         hdr->n_cell_int = 0ull;
         hdr->n_body_int = 0ull;
+        hdr->n_far_int = 0ull;
         hdr->n_groups = 0;
         for (int q = 0; q < kXcds; q++) hdr->work_next[q] = 0u;
         nd_info[0] = make_int4(-1, 0, 0, na);

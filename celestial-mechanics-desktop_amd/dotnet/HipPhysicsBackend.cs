@@ -27,6 +27,7 @@ internal static unsafe class CmHip
         public int LeafCapacity;
         public int GroupSize;
         public int LearnedCorrection; // 1: MLP far-field correction (precision 2 only)
+        public double FarFieldRatio;  // 0 off; kappa <= 0.5: group Taylor expansion of far cells (precision 2)
     }
 
     [StructLayout(LayoutKind.Sequential)]

@@ -72,6 +72,13 @@ typedef struct {
     int32_t group_size;        /* max bodies per target group (one wavefront), 1..64 */
     int32_t learned_correction;/* 1: add the MLP far-field correction (Barnes-Hut,
                                   FP32_SPLIT only; no reference counterpart) */
+    double far_field_ratio;    /* kappa in [0, 0.5]: accepted cells farther than
+                                  R/kappa from a target group's centre (R its
+                                  half diagonal) are summed for the group as a
+                                  second-order Taylor expansion of their
+                                  monopole field (Barnes-Hut FP32_SPLIT without
+                                  the learned correction; 0 = off; no reference
+                                  counterpart) */
 } cm_backend_config;
 
 /* Pointers to the 18 BodySoA arrays (SoA/BodySoA.cs:42-69).  On upload a
@@ -106,6 +113,8 @@ typedef struct {
     double acc_ms_force;        /* summed force-kernel time, HIP events */
     double acc_ms_total;        /* summed force-evaluation time */
     int32_t acc_evaluations;
+    int64_t far_cells;          /* (group, cell) far-field expansion terms */
+    int64_t acc_far_cells;      /* the same, accumulated */
 } cm_force_stats;
 
 /* EnergyCalculator quantities (Validation/EnergyCalculator.cs:16-81) with

@@ -827,12 +827,64 @@ static int32_t* make_groups(int na, const uint64_t* ks, const int32_t* ord,
     return r;
 }
 
+/* Far-field expansion of a cell about a group centre c (this build's
+ * definition; no reference counterpart).  The softened monopole field of
+ * mass M at c + r, seen at c + d, is f(d) = M (r - d) / s(r - d)^(3/2) with
+ * s(y) = |y|^2 + eps^2; its Taylor coefficients at d = 0 are
+ *   T0_a   = M r_a s^-3/2
+ *   T1_ab  = 3 M r_a r_b s^-5/2 - delta_ab M s^-3/2
+ *   T2_abc = 15 M r_a r_b r_c s^-7/2 - 3 M (delta_ab r_c + delta_ac r_b + delta_bc r_a) s^-5/2
+ * (s = |r|^2 + eps^2), and f(d) ~ T0 + T1 d + (1/2) T2 d d. */
+void cmo_far_coeffs(double M, double rx, double ry, double rz, double eps2, double* t) {
+    double s = rx * rx + ry * ry + rz * rz + eps2;
+    double ir = 1.0 / sqrt(s), ir2 = ir * ir;
+    double i3 = M * ir * ir2, i5 = i3 * ir2, i7 = i5 * ir2;
+    double r[3] = {rx, ry, rz};
+    for (int a = 0; a < 3; a++) t[a] += i3 * r[a];
+    /* T1: xx, yy, zz, xy, xz, yz */
+    static const int p1[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {0, 2}, {1, 2}};
+    for (int k = 0; k < 6; k++) {
+        int a = p1[k][0], b = p1[k][1];
+        t[3 + k] += 3.0 * i5 * r[a] * r[b] - (a == b ? i3 : 0.0);
+    }
+    /* T2: xxx, yyy, zzz, xxy, xxz, yyx, yyz, zzx, zzy, xyz */
+    static const int p2[10][3] = {{0, 0, 0}, {1, 1, 1}, {2, 2, 2}, {0, 0, 1}, {0, 0, 2},
+                                  {1, 1, 0}, {1, 1, 2}, {2, 2, 0}, {2, 2, 1}, {0, 1, 2}};
+    for (int k = 0; k < 10; k++) {
+        int a = p2[k][0], b = p2[k][1], c = p2[k][2];
+        double dl = (a == b ? r[c] : 0.0) + (a == c ? r[b] : 0.0) + (b == c ? r[a] : 0.0);
+        t[9 + k] += 15.0 * i7 * r[a] * r[b] * r[c] - 3.0 * i5 * dl;
+    }
+}
+
+void cmo_far_eval(const double* t, double dx, double dy, double dz, double* out) {
+    /* full symmetric tensors from the packed coefficients */
+    double T1[3][3] = {{t[3], t[6], t[7]}, {t[6], t[4], t[8]}, {t[7], t[8], t[5]}};
+    double T2[3][3][3];
+    static const int p2[10][3] = {{0, 0, 0}, {1, 1, 1}, {2, 2, 2}, {0, 0, 1}, {0, 0, 2},
+                                  {1, 1, 0}, {1, 1, 2}, {2, 2, 0}, {2, 2, 1}, {0, 1, 2}};
+    for (int k = 0; k < 10; k++) {
+        int i = p2[k][0], j = p2[k][1], l = p2[k][2];
+        T2[i][j][l] = T2[i][l][j] = T2[j][i][l] = T2[j][l][i] = T2[l][i][j] = T2[l][j][i] = t[9 + k];
+    }
+    double d[3] = {dx, dy, dz};
+    for (int a = 0; a < 3; a++) {
+        double o = t[a];
+        for (int b = 0; b < 3; b++) {
+            o += T1[a][b] * d[b];
+            for (int c = 0; c < 3; c++) o += 0.5 * T2[a][b][c] * d[b] * d[c];
+        }
+        out[a] = o;
+    }
+}
+
 int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
                 const double* m, const double* rad, const uint8_t* act,
                 double eps, double theta, int shell, int levels, int leaf_cap,
                 int group_size, int split_depth, int lifo, int nthreads,
-                double* ax, double* ay, double* az,
-                int32_t* n_cells, int32_t* n_parts, int32_t* max_queue) {
+                double far_ratio, double* ax, double* ay, double* az,
+                int32_t* n_cells, int32_t* n_parts, int32_t* max_queue,
+                int64_t* far_total) {
     double box[4];
     int na, nn;
     uint64_t* ks;
@@ -865,11 +917,12 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
         }
     }
     int mq = 0;
+    int64_t nfar_all = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
     (void)nthreads;
-#pragma omp parallel reduction(max : mq)
+#pragma omp parallel reduction(max : mq) reduction(+ : nfar_all)
     {
         int qcap = 1024, ccap = 1024, pcap = 1024;
         int32_t* q = (int32_t*)malloc((size_t)qcap * 4);
@@ -883,6 +936,17 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
             group_box(ord, px, py, pz, gs, ge, glo, ghi);
             double lox = glo[0], loy = glo[1], loz = glo[2];
             double hix = ghi[0], hiy = ghi[1], hiz = ghi[2];
+            /* far field: an accepted cell whose centre of mass lies beyond
+             * R / far_ratio of the box centre (R the half diagonal) joins the
+             * group's Taylor expansion instead of the per-body cell list */
+            double gcx = (lox + hix) * 0.5, gcy = (loy + hiy) * 0.5, gcz = (loz + hiz) * 0.5;
+            double far_lim = DBL_MAX;
+            if (far_ratio > 0.0) {
+                double ex = hix - lox, ey = hiy - loy, ez = hiz - loz;
+                far_lim = 0.25 * (ex * ex + ey * ey + ez * ez) / (far_ratio * far_ratio);
+            }
+            double tfar[19] = {0};
+            int nfar = 0;
             int qh = 0, qt = 0, nc = 0, np = 0;
             q[qt++] = 0;
             while (qh < qt) {
@@ -919,7 +983,15 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
                         double hl = ldexp(h, -t->level);
                         double size2 = 4.0 * hl * hl;
                         int leaf = t->first < 0;
+                        int far = 0;
                         if (!(shell && leaf) && d2 > 0.0 && size2 < theta2 * d2) {
+                            double rx = t->cx - gcx, ry = t->cy - gcy, rz = t->cz - gcz;
+                            far = rx * rx + ry * ry + rz * rz > far_lim;
+                        }
+                        if (far) {
+                            cmo_far_coeffs(t->mass, t->cx - gcx, t->cy - gcy, t->cz - gcz, eps2, tfar);
+                            nfar++;
+                        } else if (!(shell && leaf) && d2 > 0.0 && size2 < theta2 * d2) {
                             if (nc == ccap) { ccap *= 2; cl = (int32_t*)realloc(cl, (size_t)ccap * 4); }
                             cl[nc++] = kk;
                         } else if (leaf) {
@@ -945,7 +1017,15 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
                 double hl = ldexp(h, -t->level);
                 double size2 = 4.0 * hl * hl;
                 int leaf = t->first < 0;
+                int far = 0;
                 if (!(shell && leaf) && d2 > 0.0 && size2 < theta2 * d2) {
+                    double rx = t->cx - gcx, ry = t->cy - gcy, rz = t->cz - gcz;
+                    far = rx * rx + ry * ry + rz * rz > far_lim;
+                }
+                if (far) {
+                    cmo_far_coeffs(t->mass, t->cx - gcx, t->cy - gcy, t->cz - gcz, eps2, tfar);
+                    nfar++;
+                } else if (!(shell && leaf) && d2 > 0.0 && size2 < theta2 * d2) {
                     if (nc == ccap) { ccap *= 2; cl = (int32_t*)realloc(cl, (size_t)ccap * 4); }
                     cl[nc++] = k;
                 } else if (leaf) {
@@ -994,10 +1074,16 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
                     double f = cmo_gravity_coeff(dist2, m[j], rad[j], eps2, shell);
                     pxs -= f * dx; pys -= f * dy; pzs -= f * dz;
                 }
+                if (nfar) {
+                    double o[3];
+                    cmo_far_eval(tfar, xi - gcx, yi - gcy, zi - gcz, o);
+                    cxs += o[0]; cys += o[1]; czs += o[2];
+                }
                 ax[i] = cxs + pxs; ay[i] = cys + pys; az[i] = czs + pzs;
                 if (n_cells) n_cells[i] = nc;
                 if (n_parts) n_parts[i] = np - self;
             }
+            nfar_all += nfar;
             __atomic_fetch_add(&g_walk[2], (int64_t)((nc + 63) / 64) * 64, __ATOMIC_RELAXED);
             __atomic_fetch_add(&g_walk[3], (int64_t)((np + 63) / 64) * 64, __ATOMIC_RELAXED);
           }
@@ -1005,6 +1091,7 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
         free(q); free(cl); free(pl);
     }
     if (max_queue) *max_queue = mq;
+    if (far_total) *far_total = nfar_all;
     free(ks); free(ord); free(nd); free(rng);
     return nn;
 }

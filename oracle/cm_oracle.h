@@ -10,7 +10,8 @@
  *   (a) the reference's C# FP64 physics path (file:line cited per function), or
  *   (b) a definition this build adds on top of it (30-bit/63-bit Morton keys,
  *       stable key sort, the level-order bucket octree, the group-MAC
- *       traversal, Kahan reductions, the correctly-rounded cube root),
+ *       traversal, its group far-field Taylor expansion, Kahan reductions,
+ *       the correctly-rounded cube root),
  * in plain C with IEEE double arithmetic and no FMA contraction
  * (-ffp-contract=off), so that results are bit-reproducible.
  */
@@ -80,8 +81,18 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
                 const double* m, const double* rad, const uint8_t* act,
                 double eps, double theta, int shell, int levels, int leaf_cap,
                 int group_size, int split_depth, int lifo /* stack capacity; 0 = FIFO */, int nthreads,
+                double far_ratio /* 0 = no far-field expansion */,
                 double* ax, double* ay, double* az,
-                int32_t* n_cells, int32_t* n_parts, int32_t* max_queue);
+                int32_t* n_cells, int32_t* n_parts, int32_t* max_queue,
+                int64_t* far_total /* (group, cell) expansion terms; may be NULL */);
+/* Far-field expansion of one cell (this build's definition, no reference
+ * counterpart): adds to t[19] the Taylor coefficients about a group centre
+ * of the softened monopole field of mass M at offset r = (rx, ry, rz) from
+ * it: T0 (x, y, z), T1 (xx, yy, zz, xy, xz, yz), T2 (xxx, yyy, zzz, xxy,
+ * xxz, yyx, yyz, zzx, zzy, xyz).  cmo_far_eval evaluates
+ * T0 + T1 d + (1/2) T2 d d at offset d. */
+void cmo_far_coeffs(double M, double rx, double ry, double rz, double eps2, double* t19);
+void cmo_far_eval(const double* t19, double dx, double dy, double dz, double* out3);
 
 /* ---- diagnostics -------------------------------------------------------- */
 typedef struct {

@@ -94,8 +94,12 @@ def lib():
                                  i32p, i32p, i32p, f64p, f64p, i32p]
     L.cmo_tree_build.restype = I
     L.cmo_bh_tree.argtypes = [I, f64p, f64p, f64p, f64p, f64p, u8p, D, D, I, I,
-                              I, I, I, I, I, f64p, f64p, f64p, i32p, i32p,
-                              C.POINTER(C.c_int32)]
+                              I, I, I, I, I, D, f64p, f64p, f64p, i32p, i32p,
+                              C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+    L.cmo_far_coeffs.argtypes = [D, D, D, D, D, f64p]
+    L.cmo_far_coeffs.restype = V
+    L.cmo_far_eval.argtypes = [f64p, D, D, D, f64p]
+    L.cmo_far_eval.restype = V
     L.cmo_bh_tree.restype = I
     L.cmo_energy_naive.argtypes = [I, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
                                    u8p, D, I, C.POINTER(Energy)]
@@ -267,21 +271,36 @@ def tree_build(s, levels, leaf_cap):
 
 
 def bh_tree(s, eps, theta, levels=10, leaf_cap=16, group_size=64, shell=False,
-            split_depth=-3, lifo=1536, nthreads=0):
+            split_depth=-3, lifo=1536, nthreads=0, far_ratio=0.0):
     n = len(s["px"])
     ax, ay, az = (np.zeros(n) for _ in range(3))
     nc = np.zeros(n, dtype=np.int32)
     npart = np.zeros(n, dtype=np.int32)
     mq = C.c_int32(0)
+    nfar = C.c_int64(0)
     rc = lib().cmo_bh_tree(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
                            _c(s["pz"], np.float64), _c(s["mass"], np.float64),
                            _c(s["radius"], np.float64), _c(s["active"], np.uint8),
                            eps, theta, int(shell), levels, leaf_cap, group_size,
-                           split_depth, int(lifo), nthreads, ax, ay, az, nc, npart,
-                           C.byref(mq))
+                           split_depth, int(lifo), nthreads, float(far_ratio), ax, ay, az,
+                           nc, npart, C.byref(mq), C.byref(nfar))
     if rc < 0:
         raise RuntimeError("oracle target groups do not partition the active bodies")
-    return (ax, ay, az), {"n_cells": nc, "n_parts": npart, "max_queue": mq.value}
+    return (ax, ay, az), {"n_cells": nc, "n_parts": npart, "max_queue": mq.value,
+                          "far_cells": nfar.value}
+
+
+def far_coeffs(M, r, eps2, t=None):
+    """Adds the far-field Taylor coefficients of one cell to t (19 values)."""
+    t = np.zeros(19) if t is None else t
+    lib().cmo_far_coeffs(float(M), float(r[0]), float(r[1]), float(r[2]), float(eps2), t)
+    return t
+
+
+def far_eval(t, d):
+    out = np.zeros(3)
+    lib().cmo_far_eval(_c(t, np.float64), float(d[0]), float(d[1]), float(d[2]), out)
+    return out
 
 
 def energy_naive(s, eps, with_pe=True):
