@@ -299,20 +299,16 @@ struct Staging { double4 v[kWave]; double r[kWave]; int id[kWave]; };
 template <>
 struct Staging<CM_PREC_FP32_SPLIT> { float4 v[kWave]; float r[kWave]; int id[kWave]; };
 
-#ifndef CM_SPLIT_STACK
-#define CM_SPLIT_STACK 1536
-#endif
-#ifndef CM_SPLIT_WAVES
-#define CM_SPLIT_WAVES 4
-#endif
-// Occupancy of the traversal: the FP32 split kernel keeps a smaller LDS stack
-// and a register budget for more wavefronts per SIMD; the FP64 kernels keep
-// the stack the CPU restatement of this traversal models (kStackCap).
+// Occupancy of the traversal: the FP32 split kernel is held to the register
+// budget of four wavefronts per SIMD (16 per CU, as many as its LDS allows;
+// five or six per SIMD with a smaller stack measured slower); the others
+// take what they need.  All keep the stack the CPU restatement of this
+// traversal models (kStackCap), which fixes the order of the FP64 sums.
 template <int PREC, bool LEARNED>
 struct TravShape {
     static constexpr bool split = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
-    static constexpr int stack = split ? CM_SPLIT_STACK : kStackCap;
-    static constexpr int waves = split ? CM_SPLIT_WAVES : 1;   // minimum per SIMD
+    static constexpr int stack = kStackCap;
+    static constexpr int waves = split ? 4 : 1;   // minimum per SIMD
 };
 
 template <int PREC, bool LEARNED>
@@ -922,6 +918,7 @@ int launch_barnes_hut(Engine& e, double softening) {
     // never more than bodies / 8
     const bool split = e.cfg.precision == CM_PREC_FP32_SPLIT && !learned;
     const int per_simd = split ? TravShape<CM_PREC_FP32_SPLIT, false>::waves : 4;
+    static_assert(TravShape<CM_PREC_FP32_SPLIT, false>::waves == 4, "grid sized for 4 per SIMD");
     int nblocks = std::max(1, std::min(1024 * per_simd, (n + 7) / 8));
     switch (e.cfg.precision) {
         case CM_PREC_FP64:
