@@ -403,6 +403,9 @@ def merger(args, cmhip):
         log(f"merger theta={theta} plain ...")
         plain = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup,
                          eps, dt, sample)
+        log(f"merger theta={theta} with the group far field (ratio {args.far_field_ratio}) ...")
+        farf = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup,
+                        eps, dt, sample, far=args.far_field_ratio)
         log(f"merger theta={theta}: training on a 65,536-body subsample ...")
         params, tr = train_network(cmhip, soa, theta, eps)
         learned = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup,
@@ -410,6 +413,9 @@ def merger(args, cmhip):
         res[str(theta)] = {
             "plain": {k: plain[k] for k in ("body_steps_per_s", "ms_per_step",
                                             "force_err_median", "interactions_per_body")},
+            "far_field": {k: farf[k] for k in ("body_steps_per_s", "ms_per_step",
+                                               "force_err_median", "interactions_per_body",
+                                               "far_cells_per_eval")},
             "learned": {k: learned[k] for k in ("body_steps_per_s", "ms_per_step",
                                                 "force_err_median", "interactions_per_body")},
             "training": tr}
@@ -425,7 +431,7 @@ def merger(args, cmhip):
             "data": "synthetic",
             "config": {"workload": "two_plummer_merger_bh_verlet", "n_bodies": n,
                        "softening": eps, "dt": dt, "separation_au": 10.0,
-                       "speed_fraction_of_escape": 0.3, "seed": 42},
+                       "speed_fraction_of_escape": 0.3, "far_field_ratio": 0.0, "seed": 42},
             "fp64_baseline": {"value": base["body_steps_per_s"],
                               "force_err_median": base["force_err_median"]},
             "by_theta": res}

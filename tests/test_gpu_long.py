@@ -65,18 +65,21 @@ def test_two_body_circular_orbit_10k_steps(kind, prec):
 
 def test_plummer_energy_drift_10k_fp32_vs_fp64():
     # N = 4096 Plummer with softening 1e-2 (encounters resolved by dt = 1e-3):
-    # over 10k steps the FP32 split traversal drifts no more than twice the
-    # FP64 traversal (the reference's arithmetic), plus rounding noise
+    # over 10k steps the FP32 split traversal, with and without the group
+    # far-field expansion (kappa = 0.15, the benchmark's), drifts no more than
+    # twice the FP64 traversal (the reference's arithmetic), plus rounding noise
     eps, dt, n = 1e-2, 1e-3, 4096
     soa = cmhip.generate_plummer(n, seed=7, mass_fraction_cut=0.999)
     drift = {}
-    for prec in (cmhip.PREC_FP64, cmhip.PREC_FP32_SPLIT):
-        be = cmhip.HipPhysicsBackend(n, precision=prec, theta=0.5)
+    for key, prec, far in (("fp64", cmhip.PREC_FP64, 0.0), ("fp32", cmhip.PREC_FP32_SPLIT, 0.0),
+                           ("fp32_far", cmhip.PREC_FP32_SPLIT, 0.15)):
+        be = cmhip.HipPhysicsBackend(n, precision=prec, theta=0.5, far_field_ratio=far)
         be.upload(soa)
         e0 = _energy(be, eps)
         be.step(eps, dt, 10000)
-        drift[prec] = abs((_energy(be, eps) - e0) / e0)
-    print(f"N=4096 Plummer, 10k steps: |dE/E| FP64 {drift[cmhip.PREC_FP64]:.3e}, "
-          f"FP32 split {drift[cmhip.PREC_FP32_SPLIT]:.3e}")
-    assert drift[cmhip.PREC_FP32_SPLIT] <= 2.0 * drift[cmhip.PREC_FP64] + 1e-6
-    assert drift[cmhip.PREC_FP64] < 1e-3
+        drift[key] = abs((_energy(be, eps) - e0) / e0)
+    print(f"N=4096 Plummer, 10k steps: |dE/E| FP64 {drift['fp64']:.3e}, "
+          f"FP32 split {drift['fp32']:.3e}, FP32 split + far field {drift['fp32_far']:.3e}")
+    assert drift["fp32"] <= 2.0 * drift["fp64"] + 1e-6
+    assert drift["fp32_far"] <= 2.0 * drift["fp64"] + 1e-6
+    assert drift["fp64"] < 1e-3
