@@ -546,10 +546,10 @@ k_traverse(TravArgs a) {
                     if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                         st.v[slot] = make_float4((float)(sx[q] - cxg), (float)(sy[q] - cyg),
                                                  (float)(sz[q] - czg), (float)sm[q]);
-                        st.r[lane] = (float)sr[q];
+                        st.r[lane] = shell ? (float)sr[q] : 0.0f;
                     } else {
                         st.v[lane] = make_double4(sx[q], sy[q], sz[q], sm[q]);
-                        st.r[lane] = sr[q];
+                        st.r[lane] = shell ? sr[q] : 0.0;
                     }
                     st.id[lane] = q;
                 }

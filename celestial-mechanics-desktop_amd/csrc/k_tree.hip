@@ -284,6 +284,8 @@ k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
     }
 }
 
+// Sorted copies of x, y, z, m and, when the shell theorem needs source radii
+// (r != nullptr), of the radius.
 __global__ void __launch_bounds__(256)
 k_gather(int n, const int32_t* __restrict__ order, const double4* __restrict__ pm4,
          const double* __restrict__ r, double* __restrict__ sorted, int cap) {
@@ -295,7 +297,7 @@ k_gather(int n, const int32_t* __restrict__ order, const double4* __restrict__ p
     sorted[(size_t)cap + k] = v.y;
     sorted[2 * (size_t)cap + k] = v.z;
     sorted[3 * (size_t)cap + k] = v.w;
-    sorted[4 * (size_t)cap + k] = r[o];
+    if (r) sorted[4 * (size_t)cap + k] = r[o];
 }
 
 // ---------------------------------------------------------- tree build ----
@@ -743,8 +745,9 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
     int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
     radix_sort_pairs(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist);
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
-    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4, e.f[RAD],
-                       e.sorted, e.capacity);
+    // source radii enter only the shell theorem's interior branch
+    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4,
+                       e.cfg.shell_theorem ? e.f[RAD] : nullptr, e.sorted, e.capacity);
     const int cap = e.cfg.leaf_capacity;
     const int ntiles = std::max(1, (n + kTreeTile - 1) / kTreeTile);
     const int tstride = tree_tiles(e.capacity);

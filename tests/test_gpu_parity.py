@@ -97,6 +97,58 @@ def test_barnes_hut_fp64_bit_exact(plummer, levels, leaf, gsize):
     assert gs["flags"] == 0
 
 
+def _with_radii(soa, seed=9):
+    """Source radii large enough that many close pairs fall inside the source
+    (GravityKernel's interior branch), zero for a tenth of the bodies."""
+    rng = np.random.default_rng(seed)
+    n = soa.count
+    r = rng.uniform(0.0, 0.02, n)
+    r[rng.random(n) < 0.1] = 0.0
+    soa.radius[:n] = r
+    return soa
+
+
+def test_barnes_hut_fp64_shell_theorem_bit_exact(plummer):
+    """Shell theorem on (BarnesHutBackend.cs:415-421): leaves are always opened and
+    body terms use the source radius; FP64 equals the oracle bit for bit."""
+    n = 20000
+    soa = _with_radii(_with_inactive(plummer(n, cut=0.999)))
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5, leaf_capacity=8,
+                                 enable_shell_theorem=True)
+    be.compute_forces(soa, EPS)
+    ref, st = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=8, group_size=64, shell=True,
+                        split_depth=-3, lifo=1536, nthreads=8)
+    _eq(_acc(soa), ref)
+    gs = be.stats()
+    assert gs["cell_interactions"] == int(st["n_cells"].astype(np.int64).sum())
+    assert gs["body_interactions"] == int(st["n_parts"].astype(np.int64).sum())
+    # the interior branch is exercised: radii change the result
+    plain, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=8, group_size=64, shell=False,
+                         split_depth=-3, lifo=1536, nthreads=8)
+    assert np.count_nonzero(ref[0] != plain[0]) > n // 10
+
+
+def test_barnes_hut_fp32_split_shell_theorem(plummer):
+    """FP32 split with the shell theorem (and the far field, which only takes
+    accepted internal nodes) against the oracle: median <= 2e-6, max <= 1e-3."""
+    n = 20000
+    soa = _with_radii(plummer(n, cut=0.999))
+    s = soa_to_state(soa)
+    for far in (0.0, 0.15):
+        work = soa.copy()
+        be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP32_SPLIT, theta=0.5,
+                                     leaf_capacity=8, enable_shell_theorem=True,
+                                     far_field_ratio=far)
+        be.compute_forces(work, EPS)
+        ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=8, group_size=64, shell=True,
+                           split_depth=-3, lifo=1536, nthreads=8, far_ratio=far)
+        r = _rel(_acc(work), ref)
+        assert np.median(r) <= 2e-6, (far, np.median(r))
+        assert r.max() <= 1e-3, (far, r.max())
+        be.close()
+
+
 def test_barnes_hut_fp32_split_vs_fp64(plummer):
     """FP32 split precision on the same tree and lists as the FP64 path:
     per-body relative difference median <= 1e-6, max <= 1e-3."""
