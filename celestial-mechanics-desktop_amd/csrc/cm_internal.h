@@ -110,6 +110,9 @@ struct Engine {
 
     // timing
     bool timing = true;
+    // a run of several force evaluations copies the tree header back to the
+    // host (for the force statistics) after its last evaluation only
+    bool header_copy = true;
     hipEvent_t ev[6] = {};
     cm_force_stats stats{};
     // per-evaluation (start, force-kernel start, end) events since the last

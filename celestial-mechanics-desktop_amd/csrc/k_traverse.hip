@@ -131,11 +131,6 @@ k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
 // the tile counts, then write each tile's groups at its offset (ballot ranks).
 constexpr int kGroupTile = 1024;
 
-__global__ void __launch_bounds__(256)
-k_fill_i32(int n, int32_t* __restrict__ p) {
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = 0;
-}
-
 __global__ void __launch_bounds__(kGroupTile)
 k_group_count(const TreeHeader* __restrict__ hdr, const int32_t* __restrict__ gmark,
               int32_t* __restrict__ tcount) {
@@ -214,6 +209,7 @@ struct TravArgs {
     const int2* groups;
     const double* sorted;    // x, y, z, m, r blocks of `cap`
     int cap;
+    int n;                   // bodies (active first, then inactive, in sorted order)
     double* ax;
     double* ay;
     double* az;
@@ -836,6 +832,12 @@ k_traverse(TravArgs a) {
             }
         }
     }
+    // inactive bodies sit after the active ones in sorted order: zero their
+    // acc (IPhysicsComputeBackend.cs:21-33)
+    for (int k = a.hdr->n_active + blockIdx.x * kWave + lane; k < a.n; k += gridDim.x * kWave) {
+        const int o = a.order[k];
+        a.ax[o] = 0.0; a.ay[o] = 0.0; a.az[o] = 0.0;
+    }
     if (lane == 0) {
         atomicAdd(&a.counters[0], n_cell);
         atomicAdd(&a.counters[1], n_body);
@@ -847,22 +849,10 @@ k_traverse(TravArgs a) {
     }
 }
 
-// Inactive bodies sit after the active ones in sorted order: zero their acc.
-__global__ void __launch_bounds__(256)
-k_zero_inactive(int n, const TreeHeader* __restrict__ hdr,
-                const int32_t* __restrict__ order, double* __restrict__ ax,
-                double* __restrict__ ay, double* __restrict__ az) {
-    int k = hdr->n_active + blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    int o = order[k];
-    ax[o] = 0.0; ay[o] = 0.0; az[o] = 0.0;
-}
-
 int form_groups(Engine& e) {
     const int n = e.count;
     const int gb = std::min(1024, std::max(1, (n + 255) / 256));
     const int tiles = std::max(1, (n + kGroupTile - 1) / kGroupTile);
-    hipLaunchKernelGGL(k_fill_i32, dim3(gb), dim3(256), 0, e.stream, n, e.gmark);
     hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, e.stream, e.hdr, e.nd_info,
                        e.nd_box, e.cfg.group_size, e.gmark);
     hipLaunchKernelGGL(k_group_count, dim3(tiles), dim3(kGroupTile), 0, e.stream, e.hdr,
@@ -903,6 +893,7 @@ int launch_barnes_hut(Engine& e, double softening) {
     a.groups = e.groups;
     a.sorted = e.sorted;
     a.cap = e.capacity;
+    a.n = n;
     a.ax = e.f[AX]; a.ay = e.f[AY]; a.az = e.f[AZ];
     a.eps2 = softening * softening;
     double th = e.cfg.theta > 0.2 ? e.cfg.theta : 0.2;   // BarnesHutBackend.cs:129
@@ -934,12 +925,11 @@ int launch_barnes_hut(Engine& e, double softening) {
                 hipLaunchKernelGGL((k_traverse<CM_PREC_FP32_SPLIT, false>), dim3(nblocks), dim3(kWave), 0, st, a);
             break;
     }
-    hipLaunchKernelGGL(k_zero_inactive, dim3((n + 255) / 256), dim3(256), 0, st, n,
-                       e.hdr, order, e.f[AX], e.f[AY], e.f[AZ]);
     CM_HIP_TRY(e, hipGetLastError());
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[5], st));
     e.mark(2);
-    CM_HIP_TRY(e, hipMemcpyAsync(e.hdr_host, e.hdr, sizeof(TreeHeader),
+    if (e.header_copy)
+        CM_HIP_TRY(e, hipMemcpyAsync(e.hdr_host, e.hdr, sizeof(TreeHeader),
                                  hipMemcpyDeviceToHost, st));
     return CM_OK;
 }

@@ -154,9 +154,11 @@ k_morton(int n, int levels, const TreeHeader* __restrict__ hdr,
          const double* __restrict__ px, const double* __restrict__ py,
          const double* __restrict__ pz, const double* __restrict__ m,
          const uint8_t* __restrict__ act, uint64_t* __restrict__ keys,
-         int32_t* __restrict__ vals, double4* __restrict__ pm4) {
+         int32_t* __restrict__ vals, double4* __restrict__ pm4,
+         int32_t* __restrict__ gmark) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    gmark[i] = 0;   // target-group marks of the traversal, set after the build
     uint64_t sentinel = (levels >= 21) ? ~0ull : ((2ull << (3 * levels)) - 1ull);
     uint64_t k = sentinel;
     double x = px[i], y = py[i], z = pz[i];
@@ -738,7 +740,7 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
     int gb = (n + 255) / 256;
     hipLaunchKernelGGL(k_morton, dim3(gb), dim3(256), 0, st, n, levels, e.hdr,
                        e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, e.keys_a, e.vals_a,
-                       e.pm4);
+                       e.pm4, e.gmark);
     if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
     int passes = (3 * levels + 1 + 7) / 8;
     uint64_t* ka = e.keys_a; uint64_t* kb = e.keys_b;
