@@ -110,9 +110,11 @@ struct Engine {
 
     // timing
     bool timing = true;
-    // a run of several force evaluations copies the tree header back to the
-    // host (for the force statistics) after its last evaluation only
-    bool header_copy = true;
+    // the force statistics describe the last evaluation of a run: only that
+    // one records the per-phase timing events and copies the tree header
+    // back to the host (each event record costs the stream a few us)
+    bool stats_eval = true;
+    bool detail_timing() const { return timing && stats_eval; }
     hipEvent_t ev[6] = {};
     cm_force_stats stats{};
     // per-evaluation (start, force-kernel start, end) events since the last

@@ -189,9 +189,9 @@ static int verlet_run(Engine& e, double softening, const double* dts, int n) {
     if (n <= 0) return CM_OK;
     int rc = launch_kick_drift(e, dts[0]);
     for (int k = 0; k < n && rc == CM_OK; k++) {
-        e.header_copy = k + 1 == n;
+        e.stats_eval = k + 1 == n;
         rc = compute_forces(e, softening);
-        e.header_copy = true;
+        e.stats_eval = true;
         if (rc != CM_OK) break;
         rc = k + 1 < n ? launch_kick_rotate_drift(e, dts[k], dts[k + 1])
                        : launch_kick_rotate(e, dts[k]);

@@ -272,7 +272,7 @@ int launch_direct(Engine& e, double softening) {
     if (n == 0) return CM_OK;
     double eps2 = softening * softening;
     int nb = (n + kTile - 1) / kTile;
-    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[0], e.stream));
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[0], e.stream));
     e.mark(0);
     e.mark(1);
     bool pairwise = e.cfg.kind == CM_BACKEND_DIRECT_PAIRWISE && !e.cfg.shell_theorem;
@@ -293,7 +293,7 @@ int launch_direct(Engine& e, double softening) {
                            e.f[MASS], e.f[RAD], e.act, e.f[AX], e.f[AY], e.f[AZ]);
     }
     CM_HIP_TRY(e, hipGetLastError());
-    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[5], e.stream));
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[5], e.stream));
     e.mark(2);
     e.stats = cm_force_stats{};
     e.stats.n_active = -1;

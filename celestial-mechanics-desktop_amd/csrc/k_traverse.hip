@@ -869,7 +869,7 @@ int launch_barnes_hut(Engine& e, double softening) {
     e.stats = cm_force_stats{};
     if (n == 0) return CM_OK;
     hipStream_t st = e.stream;
-    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[0], st));
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[0], st));
     e.mark(0);
     const uint64_t* keys = nullptr;
     const int32_t* order = nullptr;
@@ -883,7 +883,7 @@ int launch_barnes_hut(Engine& e, double softening) {
     if (rc != CM_OK) return rc;
     rc = form_groups(e);
     if (rc != CM_OK) return rc;
-    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[3], st));
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[3], st));
     e.mark(1);
     TravArgs a;
     a.hdr = e.hdr;
@@ -926,9 +926,9 @@ int launch_barnes_hut(Engine& e, double softening) {
             break;
     }
     CM_HIP_TRY(e, hipGetLastError());
-    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[5], st));
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[5], st));
     e.mark(2);
-    if (e.header_copy)
+    if (e.stats_eval)
         CM_HIP_TRY(e, hipMemcpyAsync(e.hdr_host, e.hdr, sizeof(TreeHeader),
                                  hipMemcpyDeviceToHost, st));
     return CM_OK;

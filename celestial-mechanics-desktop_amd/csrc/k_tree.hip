@@ -741,12 +741,12 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
     hipLaunchKernelGGL(k_morton, dim3(gb), dim3(256), 0, st, n, levels, e.hdr,
                        e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, e.keys_a, e.vals_a,
                        e.pm4, e.gmark);
-    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
     int passes = (3 * levels + 1 + 7) / 8;
     uint64_t* ka = e.keys_a; uint64_t* kb = e.keys_b;
     int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
     radix_sort_pairs(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist);
-    if (e.timing) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
     // source radii enter only the shell theorem's interior branch
     hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4,
                        e.cfg.shell_theorem ? e.f[RAD] : nullptr, e.sorted, e.capacity);
