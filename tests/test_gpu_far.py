@@ -105,3 +105,75 @@ def test_far_field_ratio_validation():
         with pytest.raises(cmhip.CmError):
             be.configure(far_field_ratio=bad)
     be.configure(far_field_ratio=0.5)
+
+
+def _state_of(soa):
+    return soa_to_state(soa)
+
+
+@pytest.mark.parametrize("case", ["n1", "n2", "n3", "coincident", "inactive", "empty",
+                                  "theta_clamp_max_kappa", "no_softening", "clusters"])
+def test_far_field_edge_cases(case):
+    """Small, degenerate and clustered inputs through the far-field path,
+    against the oracle's FP64 restatement (same counts, relative difference
+    median <= 2e-6, max <= 1e-3; inactive bodies get exactly 0)."""
+    rng = np.random.default_rng(17)
+    theta, kappa, eps, group, leaf = 0.5, 0.15, EPS, 64, 8
+    if case == "n1":
+        p = np.zeros((1, 3)); m = np.ones(1)
+    elif case == "n2":
+        p = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]]); m = np.array([1.0, 1e-3])
+    elif case == "n3":
+        p = rng.normal(0, 1, (3, 3)); m = rng.uniform(0.1, 1, 3)
+    elif case == "coincident":
+        # many bodies at identical positions: zero-extent groups and leaves
+        p = np.repeat(rng.normal(0, 1, (40, 3)), 50, axis=0); m = rng.uniform(0.1, 1, 2000)
+    elif case == "inactive" or case == "empty":
+        p = rng.normal(0, 1, (3000, 3)); m = rng.uniform(0.1, 1, 3000)
+    elif case == "theta_clamp_max_kappa":
+        p = rng.normal(0, 1, (5000, 3)); m = rng.uniform(0.1, 1, 5000)
+        theta, kappa = 0.0, 0.5
+    elif case == "no_softening":
+        p = rng.normal(0, 1, (5000, 3)); m = rng.uniform(0.1, 1, 5000)
+        eps, group = 0.0, 16
+    else:   # a few dense clusters far apart: far cells at every level
+        c = rng.normal(0, 100, (8, 3))
+        p = np.concatenate([c[k] + rng.normal(0, 0.01, (600, 3)) for k in range(8)])
+        m = rng.uniform(0.1, 1, len(p))
+    n = len(p)
+    act = np.ones(n, np.uint8)
+    if case == "inactive":
+        act[rng.random(n) < 0.5] = 0
+    if case == "empty":
+        act[:] = 0
+    soa = cmhip.BodySoA.from_arrays(pos_x=p[:, 0], pos_y=p[:, 1], pos_z=p[:, 2], mass=m,
+                                    acc_x=np.full(n, 7.0), is_active=act)
+    s = _state_of(soa)
+    print(f"far-field edge case {case}: n={n}", flush=True)
+    be = cmhip.HipPhysicsBackend(max(n, 16), precision=cmhip.PREC_FP32_SPLIT, theta=theta,
+                                 leaf_capacity=leaf, group_size=group, far_field_ratio=kappa)
+    be.compute_forces(soa, eps)
+    st = be.stats()
+    a = _acc(soa)
+    live = act != 0
+    for x in a:
+        assert np.all(x[~live] == 0.0)
+    if not live.any():
+        return
+    ref, rst = O.bh_tree(s, eps, theta, levels=10, leaf_cap=leaf, group_size=group,
+                         split_depth=-3, lifo=1536, nthreads=8, far_ratio=kappa)
+    assert st["far_cells"] == rst["far_cells"]
+    assert st["cell_interactions"] == int(rst["n_cells"].astype(np.int64)[live].sum())
+    assert st["body_interactions"] == int(rst["n_parts"].astype(np.int64)[live].sum())
+    ra = [x[live] for x in a]
+    rr = [x[live] for x in ref]
+    den = np.sqrt(rr[0] ** 2 + rr[1] ** 2 + rr[2] ** 2)
+    num = np.sqrt((ra[0] - rr[0]) ** 2 + (ra[1] - rr[1]) ** 2 + (ra[2] - rr[2]) ** 2)
+    nz = den > 0
+    if case == "n1":
+        assert np.all(num == 0.0) and np.all(den == 0.0)
+        return
+    r = num[nz] / den[nz]
+    assert np.all(num[~nz] == 0.0)
+    assert np.median(r) <= 2e-6, np.median(r)
+    assert r.max() <= 1e-3, r.max()
