@@ -11,8 +11,8 @@
 //   3. Stable LSD radix sort of (key, body index), 8-bit digits: per-tile
 //      LDS digit histograms, one scan, and a scatter that ranks equal digits
 //      with 64-lane ballots in original order (stable).
-//   4. k_gather: sorted copies of position / mass / radius (coalesced reads
-//      for the tree and the traversal).
+//   4. k_gather: sorted copies of position / mass (and radius, which only the
+//      shell theorem reads) for the tree and the traversal.
 //   5. k_tree_count / k_tree_scan / k_tree_emit: level-order octree from
 //      local comparisons of neighbouring keys (below).  A node with
 //      count > leaf_capacity below the deepest level has its non-empty
