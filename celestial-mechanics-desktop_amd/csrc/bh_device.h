@@ -23,9 +23,12 @@ __device__ __forceinline__ int mac_kind(double4 c, int4 nd, double lo0, double l
     if (c.w == 0.0) return 0;
     const int level = nd.y >> 8;
     const bool leaf = nd.x < 0;
-    double ddx = c.x < lo0 ? lo0 - c.x : (c.x > hi0 ? c.x - hi0 : 0.0);
-    double ddy = c.y < lo1 ? lo1 - c.y : (c.y > hi1 ? c.y - hi1 : 0.0);
-    double ddz = c.z < lo2 ? lo2 - c.z : (c.z > hi2 ? c.z - hi2 : 0.0);
+    // distance to the box per axis: (lo - x) if x < lo, (x - hi) if x > hi,
+    // else 0; written with max, which gives the same doubles (at most one of
+    // the two terms is positive, and adding +0 changes nothing)
+    double ddx = fmax(lo0 - c.x, 0.0) + fmax(c.x - hi0, 0.0);
+    double ddy = fmax(lo1 - c.y, 0.0) + fmax(c.y - hi1, 0.0);
+    double ddz = fmax(lo2 - c.z, 0.0) + fmax(c.z - hi2, 0.0);
     double d2 = ddx * ddx + ddy * ddy + ddz * ddz;
     double hl = ldexp(h, -level);
     double size2 = 4.0 * hl * hl;
