@@ -380,6 +380,22 @@ def extensions(args, cmhip):
     yo["yoshida4"]["force_evaluations_per_step"] = 3
     res["yoshida4_bh"] = yo
     log(json.dumps({"yoshida4_bh": yo}))
+    # potential energy of the same state: Barnes-Hut (the engine's
+    # compute_potential_bh_kernel, theta = 0.5) against the exact pair sum
+    b = make_backend(cmhip, nb, cmhip.PREC_FP32_SPLIT, 0.5)
+    b.upload(big)
+    pe = {}
+    for name, mode, r in (("barnes_hut", 2, 5), ("exact", 1, 1)):
+        b.energy(eps, mode)
+        t0 = time.perf_counter()
+        for _ in range(r):
+            e = b.energy(eps, mode)
+        pe[name] = {"ms": 1e3 * (time.perf_counter() - t0) / r, "potential": e["potential"]}
+    pe["barnes_hut"]["rel_err"] = abs(pe["barnes_hut"]["potential"] - pe["exact"]["potential"]) \
+        / abs(pe["exact"]["potential"])
+    b.close()
+    res["potential_energy"] = pe
+    log(json.dumps({"potential_energy": pe}))
     line = {"metric": "1PN pair terms/s, FP64 direct sum at N=65536 (k_pn)",
             "value": res["pn"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1,
             "steps": reps, "warmup": 1, "ms_per_step": res["pn"]["ms_k_pn"],
@@ -389,7 +405,7 @@ def extensions(args, cmhip):
                        "velocity_sigma_c": 0.01, "seed": 42},
             "roofline": res["pn"]["roofline"], "cpu_baseline": res["pn"]["cpu_baseline"],
             "softening_modes": {k: res[k] for k in ("global", "per_type", "adaptive")},
-            "yoshida4_bh": yo}
+            "yoshida4_bh": yo, "potential_energy_1m": pe}
     print(json.dumps(line), flush=True)
 
 

@@ -48,6 +48,7 @@ ALL_FIELDS = F64_FIELDS + U8_FIELDS + I32_FIELDS
 EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_default_config", "cm_upload_soa", "cm_download_soa",
            "cm_get_count", "cm_compute_forces", "cm_verlet_step", "cm_energy",
+           "cm_potential_rows",
            "cm_direct_sample", "cm_get_force_stats", "cm_detect_collisions",
            "cm_resolve_merges", "cm_compact", "cm_generate_plummer",
            "cm_reset_force_stats", "cm_generate_disk", "cm_ensemble_create",
@@ -177,6 +178,7 @@ def load_library():
         "cm_compute_forces": ([vp, f64], i32),
         "cm_verlet_step": ([vp, f64, f64, i32], i32),
         "cm_energy": ([vp, f64, i32, C.POINTER(EnergyReport)], i32),
+        "cm_potential_rows": ([vp, f64, i32, C.POINTER(C.c_double)], i32),
         "cm_direct_sample": ([vp, f64, vp, i32, vp, vp, vp], i32),
         "cm_get_force_stats": ([vp, C.POINTER(ForceStats)], i32),
         "cm_reset_force_stats": ([vp], i32),
@@ -480,6 +482,7 @@ class HipPhysicsBackend:
         self._check(self._L.cm_reset_force_stats(self._h), "cm_reset_force_stats")
 
     def energy(self, softening, with_potential=True):
+        """with_potential: False/0 none, True/1 exact pair sum, 2 Barnes-Hut."""
         r = EnergyReport()
         self._check(self._L.cm_energy(self._h, float(softening), int(with_potential), C.byref(r)),
                     "cm_energy")
@@ -487,6 +490,16 @@ class HipPhysicsBackend:
                 "momentum": tuple(r.momentum), "angular_momentum": tuple(r.angular_momentum),
                 "angular_momentum_mag": r.angular_momentum_mag,
                 "total_mass": r.total_mass, "active_count": r.active_count}
+
+    def potential_rows(self, softening, method=2):
+        """Per-body shares 0.5 m_i phi_i of the potential energy (method 1 exact,
+        2 Barnes-Hut), in body order."""
+        n = self.count
+        rows = np.zeros(max(n, 1), dtype=np.float64)
+        self._check(self._L.cm_potential_rows(self._h, float(softening), int(method),
+                                              rows.ctypes.data_as(C.POINTER(C.c_double))),
+                    "cm_potential_rows")
+        return rows[:n]
 
     def direct_sample(self, softening, targets):
         t = np.ascontiguousarray(targets, dtype=np.int32)

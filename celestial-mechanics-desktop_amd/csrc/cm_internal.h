@@ -42,6 +42,7 @@ struct TreeHeader {
     unsigned long long acc_int;   // cell + body terms since the last stats reset
     unsigned long long n_far_int; // (group, cell) far-field expansion terms
     unsigned long long acc_far;   // the same since the last stats reset
+    unsigned long long pot_int[5];   // terms counted by potential evaluations (unused)
     int32_t n_groups;             // target groups formed for the traversal
     uint32_t work_next[8];        // per-XCD queues: next group of each eighth
 };
@@ -139,6 +140,8 @@ int launch_barnes_hut(Engine& e, double softening);
 int launch_direct_sample(Engine& e, double softening, const int32_t* targets,
                          int32_t nt, double* ax, double* ay, double* az);
 int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out);
+int launch_bh_potential(Engine& e, double softening, double* rows);
+int potential_rows(Engine& e, double softening, int method, double* rows);
 int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne);
 int collide_resolve(Engine& e, const cm_collision_config& cfg, int32_t* merged);
 int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index);

@@ -526,8 +526,20 @@ int32_t cm_energy(void* h, double softening, int32_t with_potential,
     Engine* e = as_engine(h);
     if (!e) return CM_ERR_NOT_INITIALIZED;
     if (!out) return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "null output");
+    if (with_potential < 0 || with_potential > 2)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "with_potential must be 0, 1 or 2");
     CM_HIP_TRY(*e, hipSetDevice(e->device));
     return cm::launch_energy(*e, softening, with_potential, out);
+}
+
+int32_t cm_potential_rows(void* h, double softening, int32_t method, double* rows) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (method != 1 && method != 2)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "method must be 1 or 2");
+    if (!rows && e->count > 0) return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "null rows");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    return cm::potential_rows(*e, softening, method, rows);
 }
 
 int32_t cm_direct_sample(void* h, double softening, const int32_t* targets,

@@ -9,7 +9,9 @@
 // engine/src/cuda/gpu_reduction.cu:21-39), in a fixed launch shape, so the
 // results are deterministic and accurate to a few ulps of the exact sums.
 // PE is formed per body as a full row over j != i (tiled through LDS) and
-// halved at the end.
+// halved at the end, or (with_pe = 2) from each body's Barnes-Hut potential,
+// 0.5 m_i phi_i, as the engine's launch_gpu_potential_energy does
+// (engine/src/cuda/potential_energy_kernel.cu:134-171).
 #include <hip/hip_runtime.h>
 
 #include "cm_internal.h"
@@ -138,15 +140,57 @@ k_pe_rows(int n, double eps2, const double* __restrict__ px,
     if (i < n) rows[i] = me ? 0.5 * (r.s + r.c) : 0.0;
 }
 
+// Per-body shares of the potential energy (device array of count doubles):
+// method 1 the exact row sums, method 2 the Barnes-Hut potential.
+static int launch_rows(Engine& e, double softening, int method, double* rows) {
+    const int n = e.count;
+    if (method == 1) {
+        hipLaunchKernelGGL(k_pe_rows, dim3((n + 255) / 256), dim3(256), 0, e.stream, n,
+                           softening * softening, e.f[PX], e.f[PY], e.f[PZ],
+                           e.f[MASS], e.act, rows);
+        CM_HIP_TRY(e, hipGetLastError());
+        return CM_OK;
+    }
+    return launch_bh_potential(e, softening, rows);
+}
+
+// A tree that outgrew the node pool yields no potential.
+static int check_pool(Engine& e) {
+    int32_t over = 0;
+    CM_HIP_TRY(e, hipMemcpyAsync(&over, &e.hdr->overflow, 4, hipMemcpyDeviceToHost, e.stream));
+    CM_HIP_TRY(e, hipStreamSynchronize(e.stream));
+    return over ? set_error(e, CM_ERR_POOL_EXHAUSTED, "node pool exhausted") : CM_OK;
+}
+
+int potential_rows(Engine& e, double softening, int method, double* host_rows) {
+    const int n = e.count;
+    if (n == 0) return CM_OK;
+    double* rows = nullptr;
+    CM_HIP_TRY(e, hipMalloc(&rows, (size_t)n * 8));
+    int rc = launch_rows(e, softening, method, rows);
+    if (rc == CM_OK && method == 2) rc = check_pool(e);
+    if (rc == CM_OK) {
+        hipError_t err = hipMemcpyAsync(host_rows, rows, (size_t)n * 8, hipMemcpyDeviceToHost,
+                                        e.stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+        if (err != hipSuccess) rc = set_error(e, CM_ERR_HIP, hipGetErrorString(err));
+    }
+    (void)hipFree(rows);
+    return rc;
+}
+
 int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out) {
     const int n = e.count;
     double* rows = nullptr;
     hipStream_t st = e.stream;
     if (with_pe && n > 0) {
         CM_HIP_TRY(e, hipMalloc(&rows, (size_t)n * 8));
-        hipLaunchKernelGGL(k_pe_rows, dim3((n + 255) / 256), dim3(256), 0, st, n,
-                           softening * softening, e.f[PX], e.f[PY], e.f[PZ],
-                           e.f[MASS], e.act, rows);
+        int rc = launch_rows(e, softening, with_pe, rows);
+        if (rc == CM_OK && with_pe == 2) rc = check_pool(e);
+        if (rc != CM_OK) {
+            (void)hipFree(rows);
+            return rc;
+        }
     }
     Comp* part = reinterpret_cast<Comp*>(e.red);
     double* res = e.red + 2 * kQ * kRedBlocks;

@@ -220,6 +220,7 @@ struct TravArgs {
     unsigned long long* counters;   // [0] cell, [1] body, [2] accumulated terms,
                                     // [3] far cells, [4] accumulated far cells
     double far_ratio2;              // kappa^2 of the far-field expansion (0: off)
+    double* rows;                   // potential mode: 0.5 m_i phi_i per body (original index)
     // learned far-field correction (k_traverse<FP32_SPLIT, true> only)
     const uint16_t* mlp_w;          // bf16 weights [out][in] per layer
     const float* mlp_p;             // FP32 parameters (biases)
@@ -307,9 +308,15 @@ struct TravShape {
     static constexpr int waves = split ? 4 : 1;   // minimum per SIMD
 };
 
-template <int PREC, bool LEARNED>
+// POT (FP64 only): the same walk sums the potential phi_i = -sum m / sqrt(d^2
+// + eps^2) over the cells and bodies of each body's lists instead of its
+// acceleration, as compute_potential_bh_kernel does over its own tree
+// (engine/src/cuda/potential_energy_kernel.cu:29-111), and writes
+// 0.5 m_i phi_i, body i's share of the potential energy.
+template <int PREC, bool LEARNED, bool POT = false>
 __global__ void __launch_bounds__(kWave, (TravShape<PREC, LEARNED>::waves))
 k_traverse(TravArgs a) {
+    static_assert(!POT || (PREC == CM_PREC_FP64 && !LEARNED), "potential mode is FP64");
     constexpr int kCap = TravShape<PREC, LEARNED>::stack;
     __shared__ int s_stack[kCap];
     __shared__ int s_cell[2 * kWave];
@@ -601,6 +608,18 @@ k_traverse(TravArgs a) {
                         }
                     }
                     apx += (double)gx; apy += (double)gy; apz += (double)gz;
+                } else if (POT) {
+                    // potential: phi -= m / sqrt(d^2 + eps^2), cells and
+                    // bodies (self skipped) summed apart
+                    for (int j = 0; j < cnt; j++) {
+                        if (!cells && st.id[j] == my) continue;
+                        double4 v = st.v[j];
+                        double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
+                        double dist2 = dx * dx + dy * dy + dz * dz;
+                        double dist = sqrt(dist2 + a.eps2);
+                        if (cells) acx -= v.w / dist;
+                        else apx -= v.w / dist;
+                    }
                 } else if (cells) {
                     for (int j = 0; j < cnt; j++) {
                         double4 v = st.v[j];
@@ -825,7 +844,9 @@ k_traverse(TravArgs a) {
         }
         if (member) {
             const int o = a.order[my];
-            if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+            if constexpr (POT) {
+                a.rows[o] = 0.5 * (sm[my] * (acx + apx));
+            } else if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                 a.ax[o] = apx; a.ay[o] = apy; a.az[o] = apz;
             } else {
                 a.ax[o] = acx + apx; a.ay[o] = acy + apy; a.az[o] = acz + apz;
@@ -836,7 +857,11 @@ k_traverse(TravArgs a) {
     // acc (IPhysicsComputeBackend.cs:21-33)
     for (int k = a.hdr->n_active + blockIdx.x * kWave + lane; k < a.n; k += gridDim.x * kWave) {
         const int o = a.order[k];
-        a.ax[o] = 0.0; a.ay[o] = 0.0; a.az[o] = 0.0;
+        if constexpr (POT) {
+            a.rows[o] = 0.0;
+        } else {
+            a.ax[o] = 0.0; a.ay[o] = 0.0; a.az[o] = 0.0;
+        }
     }
     if (lane == 0) {
         atomicAdd(&a.counters[0], n_cell);
@@ -931,6 +956,42 @@ int launch_barnes_hut(Engine& e, double softening) {
     if (e.stats_eval)
         CM_HIP_TRY(e, hipMemcpyAsync(e.hdr_host, e.hdr, sizeof(TreeHeader),
                                  hipMemcpyDeviceToHost, st));
+    return CM_OK;
+}
+
+// Barnes-Hut potential of every body for the current state: rows[i] =
+// 0.5 m_i phi_i (0 for inactive bodies), with the backend's theta (clamped to
+// >= 0.2) and this softening, through the FP64 group traversal on a fresh
+// tree; the force statistics are left alone.
+int launch_bh_potential(Engine& e, double softening, double* rows) {
+    const int n = e.count;
+    if (n == 0) return CM_OK;
+    const uint64_t* keys = nullptr;
+    const int32_t* order = nullptr;
+    int rc = build_tree(e, &keys, &order);
+    if (rc != CM_OK) return rc;
+    rc = form_groups(e);
+    if (rc != CM_OK) return rc;
+    TravArgs a{};
+    a.hdr = e.hdr;
+    a.info = e.nd_info;
+    a.cm = e.nd_cm;
+    a.order = order;
+    a.groups = e.groups;
+    a.sorted = e.sorted;
+    a.cap = e.capacity;
+    a.n = n;
+    a.rows = rows;
+    a.eps2 = softening * softening;
+    double th = e.cfg.theta > 0.2 ? e.cfg.theta : 0.2;
+    a.theta2 = th * th;
+    a.levels = e.cfg.morton_levels;
+    a.shell = 0;   // the potential kernel has no shell-theorem branch
+    a.counters = e.hdr->pot_int;
+    int nblocks = std::max(1, std::min(1024 * 4, (n + 7) / 8));
+    hipLaunchKernelGGL((k_traverse<CM_PREC_FP64, false, true>), dim3(nblocks), dim3(kWave), 0,
+                       e.stream, a);
+    CM_HIP_TRY(e, hipGetLastError());
     return CM_OK;
 }
 

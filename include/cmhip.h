@@ -171,9 +171,21 @@ int32_t cm_compute_forces(void* engine, double softening);
 int32_t cm_verlet_step(void* engine, double softening, double dt,
                        int32_t n_steps);
 /* EnergyCalculator.ComputeKE/ComputePE/ComputeMomentum/
- * ComputeAngularMomentum (Validation/EnergyCalculator.cs:16-81). */
+ * ComputeAngularMomentum (Validation/EnergyCalculator.cs:16-81).
+ * with_potential: 0 no PE; 1 the exact pair sum (EnergyCalculator.ComputePE,
+ * NBodySolver.cs:313-337); 2 the Barnes-Hut potential energy
+ * PE = 1/2 sum m_i phi_i of the engine's launch_gpu_potential_energy
+ * (engine/src/cuda/potential_energy_kernel.cu:134-171), phi through this
+ * library's FP64 group traversal with the configured theta (no shell
+ * theorem), on a tree built for the current positions. */
 int32_t cm_energy(void* engine, double softening, int32_t with_potential,
                   cm_energy_report* out);
+/* Per-body shares of the potential energy, rows[i] = 1/2 m_i phi_i (0 for
+ * inactive bodies; host array of count doubles): method 1 exact, method 2
+ * Barnes-Hut as in cm_energy.  The per-body m_i phi_i the engine's
+ * compute_potential_bh_kernel writes (potential_energy_kernel.cu:29-111),
+ * halved. */
+int32_t cm_potential_rows(void* engine, double softening, int32_t method, double* rows);
 /* FP64 direct-sum accelerations on a subset of target bodies: the judge of
  * ForceErrorAnalyzer.CompareForces (Validation/ForceErrorAnalyzer.cs:67-92)
  * restricted to a sample.  Results are host arrays of length n_targets. */

@@ -878,13 +878,15 @@ void cmo_far_eval(const double* t, double dx, double dy, double dz, double* out)
     }
 }
 
-int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
-                const double* m, const double* rad, const uint8_t* act,
-                double eps, double theta, int shell, int levels, int leaf_cap,
-                int group_size, int split_depth, int lifo, int nthreads,
-                double far_ratio, double* ax, double* ay, double* az,
-                int32_t* n_cells, int32_t* n_parts, int32_t* max_queue,
-                int64_t* far_total) {
+/* The group traversal; with rows != NULL it sums each body's potential
+ * instead of its acceleration (see cmo_bh_potential). */
+static int bh_tree_core(int n, const double* px, const double* py, const double* pz,
+                        const double* m, const double* rad, const uint8_t* act,
+                        double eps, double theta, int shell, int levels, int leaf_cap,
+                        int group_size, int split_depth, int lifo, int nthreads,
+                        double far_ratio, double* ax, double* ay, double* az,
+                        int32_t* n_cells, int32_t* n_parts, int32_t* max_queue,
+                        int64_t* far_total, double* rows) {
     double box[4];
     int na, nn;
     uint64_t* ks;
@@ -893,7 +895,8 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
     tree_build_internal(n, px, py, pz, m, act, levels, leaf_cap, box, &na, &ks,
                         &ord, &nd, &nn, NULL);
     for (int i = 0; i < n; i++) {
-        ax[i] = 0.0; ay[i] = 0.0; az[i] = 0.0;
+        if (rows) rows[i] = 0.0;
+        else { ax[i] = 0.0; ay[i] = 0.0; az[i] = 0.0; }
         if (n_cells) n_cells[i] = 0;
         if (n_parts) n_parts[i] = 0;
     }
@@ -1048,6 +1051,34 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
             for (int s = gs; s < ge; s++) {
                 int i = ord[s];
                 double xi = px[i], yi = py[i], zi = pz[i];
+                if (rows) {
+                    /* potential: phi -= m / sqrt(d^2 + eps^2) over the cells,
+                     * then over the bodies (self skipped); the share of the
+                     * energy is 1/2 m_i phi_i (potential_energy_kernel.cu:
+                     * 29-111, 160-165) */
+                    double pc = 0.0, pp = 0.0;
+                    for (int c = 0; c < nc; c++) {
+                        const tnode* t = &nd[cl[c]];
+                        double dx = xi - t->cx, dy = yi - t->cy, dz = zi - t->cz;
+                        double dist2 = dx * dx + dy * dy + dz * dz;
+                        double dist = sqrt(dist2 + eps2);
+                        pc -= t->mass / dist;
+                    }
+                    int self = 0;
+                    for (int c = 0; c < np; c++) {
+                        int sj = pl[c];
+                        if (sj == s) { self = 1; continue; }
+                        int j = ord[sj];
+                        double dx = xi - px[j], dy = yi - py[j], dz = zi - pz[j];
+                        double dist2 = dx * dx + dy * dy + dz * dz;
+                        double dist = sqrt(dist2 + eps2);
+                        pp -= m[j] / dist;
+                    }
+                    rows[i] = 0.5 * (m[i] * (pc + pp));
+                    if (n_cells) n_cells[i] = nc;
+                    if (n_parts) n_parts[i] = np - self;
+                    continue;
+                }
                 double cxs = 0.0, cys = 0.0, czs = 0.0;
                 for (int c = 0; c < nc; c++) {
                     const tnode* t = &nd[cl[c]];
@@ -1094,6 +1125,89 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
     if (far_total) *far_total = nfar_all;
     free(ks); free(ord); free(nd); free(rng);
     return nn;
+}
+
+int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
+                const double* m, const double* rad, const uint8_t* act,
+                double eps, double theta, int shell, int levels, int leaf_cap,
+                int group_size, int split_depth, int lifo, int nthreads,
+                double far_ratio, double* ax, double* ay, double* az,
+                int32_t* n_cells, int32_t* n_parts, int32_t* max_queue,
+                int64_t* far_total) {
+    return bh_tree_core(n, px, py, pz, m, rad, act, eps, theta, shell, levels, leaf_cap,
+                        group_size, split_depth, lifo, nthreads, far_ratio, ax, ay, az,
+                        n_cells, n_parts, max_queue, far_total, NULL);
+}
+
+/* Barnes-Hut potential through the group traversal (this build's
+ * definition of the engine's compute_potential_bh_kernel,
+ * engine/src/cuda/potential_energy_kernel.cu:29-111): rows[i] =
+ * 1/2 m_i phi_i, phi_i = -sum m / sqrt(d^2 + eps^2) over body i's cells and
+ * bodies (no shell theorem, no far field, theta clamped as the force path). */
+int cmo_bh_potential(int n, const double* px, const double* py, const double* pz,
+                     const double* m, const uint8_t* act, double eps, double theta,
+                     int levels, int leaf_cap, int group_size, int split_depth, int lifo,
+                     int nthreads, double* rows, int32_t* n_cells, int32_t* n_parts) {
+    return bh_tree_core(n, px, py, pz, m, NULL, act, eps, theta, 0, levels, leaf_cap,
+                        group_size, split_depth, lifo, nthreads, 0.0, NULL, NULL, NULL,
+                        n_cells, n_parts, NULL, NULL, rows);
+}
+
+/* compute_potential_bh_kernel (engine/src/cuda/potential_energy_kernel.cu:
+ * 29-111) restated over the C# insertion octree: zero-mass nodes skipped,
+ * a leaf (one body) gives -m / sqrt(d^2 + eps^2) unless it is the body
+ * itself, an internal node with d^2 > 0 and (2h)^2 < theta^2 d^2 its
+ * monopole, else its children 0..7 in order; theta is used as given. */
+static void cs_potential(const csnode* nodes, int32_t ni, int32_t b, double bx, double by,
+                         double bz, double eps2, double theta2, double* phi) {
+    const csnode* nd = &nodes[ni];
+    if (nd->mass <= 0.0) return;
+    double dx = bx - nd->comx, dy = by - nd->comy, dz = bz - nd->comz;
+    double dist2 = dx * dx + dy * dy + dz * dz;
+    if (nd->leaf) {
+        if (nd->body == b) return;
+        *phi -= nd->mass / sqrt(dist2 + eps2);
+        return;
+    }
+    double size2 = 4.0 * nd->half * nd->half;
+    if (dist2 > 0.0 && size2 < theta2 * dist2) {
+        *phi -= nd->mass / sqrt(dist2 + eps2);
+        return;
+    }
+    for (int k = 0; k < 8; k++)
+        if (nd->child[k] != -1) cs_potential(nodes, nd->child[k], b, bx, by, bz, eps2, theta2, phi);
+}
+
+int cmo_bh_potential_reference(int n, const double* px, const double* py, const double* pz,
+                               const double* m, const uint8_t* act, double eps, double theta,
+                               int nthreads, double* rows) {
+    for (int i = 0; i < n; i++) rows[i] = 0.0;
+    if (n == 0) return 0;
+    double box[4];
+    cmo_root_box(n, px, py, pz, act, box);
+    cspool pool;
+    pool.cap = 4 * (int64_t)n + 64;
+    pool.nodes = (csnode*)malloc((size_t)pool.cap * sizeof(csnode));
+    if (!pool.nodes) return -1;
+    pool.count = 0;
+    pool.min_node_size = 1e-12;
+    int32_t root = cs_alloc(&pool, box[0], box[1], box[2], box[3]);
+    for (int i = 0; i < n; i++)
+        if (act[i]) cs_insert(&pool, root, i, px[i], py[i], pz[i], m[i]);
+    double eps2 = eps * eps, theta2 = theta * theta;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    (void)nthreads;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        if (!act[i]) continue;
+        double phi = 0.0;
+        cs_potential(pool.nodes, root, i, px[i], py[i], pz[i], eps2, theta2, &phi);
+        rows[i] = 0.5 * (m[i] * phi);
+    }
+    free(pool.nodes);
+    return 0;
 }
 
 /* ======================================================================== */

@@ -93,6 +93,16 @@ int cmo_bh_tree(int n, const double* px, const double* py, const double* pz,
  * T0 + T1 d + (1/2) T2 d d at offset d. */
 void cmo_far_coeffs(double M, double rx, double ry, double rz, double eps2, double* t19);
 void cmo_far_eval(const double* t19, double dx, double dy, double dz, double* out3);
+/* Barnes-Hut potential shares rows[i] = 1/2 m_i phi_i: through the group
+ * traversal (this build's definition), and as the engine's
+ * compute_potential_bh_kernel over the C# insertion octree. */
+int cmo_bh_potential(int n, const double* px, const double* py, const double* pz,
+                     const double* m, const uint8_t* act, double eps, double theta,
+                     int levels, int leaf_cap, int group_size, int split_depth, int lifo,
+                     int nthreads, double* rows, int32_t* n_cells, int32_t* n_parts);
+int cmo_bh_potential_reference(int n, const double* px, const double* py, const double* pz,
+                               const double* m, const uint8_t* act, double eps, double theta,
+                               int nthreads, double* rows);
 
 /* ---- diagnostics -------------------------------------------------------- */
 typedef struct {

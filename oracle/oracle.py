@@ -96,6 +96,11 @@ def lib():
     L.cmo_bh_tree.argtypes = [I, f64p, f64p, f64p, f64p, f64p, u8p, D, D, I, I,
                               I, I, I, I, I, D, f64p, f64p, f64p, i32p, i32p,
                               C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+    L.cmo_bh_potential.argtypes = [I, f64p, f64p, f64p, f64p, u8p, D, D, I, I, I, I, I, I,
+                                   f64p, i32p, i32p]
+    L.cmo_bh_potential.restype = I
+    L.cmo_bh_potential_reference.argtypes = [I, f64p, f64p, f64p, f64p, u8p, D, D, I, f64p]
+    L.cmo_bh_potential_reference.restype = I
     L.cmo_far_coeffs.argtypes = [D, D, D, D, D, f64p]
     L.cmo_far_coeffs.restype = V
     L.cmo_far_eval.argtypes = [f64p, D, D, D, f64p]
@@ -288,6 +293,33 @@ def bh_tree(s, eps, theta, levels=10, leaf_cap=16, group_size=64, shell=False,
         raise RuntimeError("oracle target groups do not partition the active bodies")
     return (ax, ay, az), {"n_cells": nc, "n_parts": npart, "max_queue": mq.value,
                           "far_cells": nfar.value}
+
+
+def bh_potential(s, eps, theta, levels=10, leaf_cap=16, group_size=64, split_depth=-3,
+                 lifo=1536, nthreads=0):
+    """Per-body shares 1/2 m_i phi_i of the Barnes-Hut potential energy through
+    the group traversal, with the per-body term counts."""
+    n = len(s["px"])
+    rows = np.zeros(n)
+    nc = np.zeros(n, dtype=np.int32)
+    npart = np.zeros(n, dtype=np.int32)
+    rc = lib().cmo_bh_potential(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                                _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                                _c(s["active"], np.uint8), eps, theta, levels, leaf_cap,
+                                group_size, split_depth, int(lifo), nthreads, rows, nc, npart)
+    if rc < 0:
+        raise RuntimeError("oracle target groups do not partition the active bodies")
+    return rows, {"n_cells": nc, "n_parts": npart}
+
+
+def bh_potential_reference(s, eps, theta, nthreads=0):
+    """compute_potential_bh_kernel's per-body traversal over the C# octree."""
+    n = len(s["px"])
+    rows = np.zeros(n)
+    lib().cmo_bh_potential_reference(n, _c(s["px"], np.float64), _c(s["py"], np.float64),
+                                     _c(s["pz"], np.float64), _c(s["mass"], np.float64),
+                                     _c(s["active"], np.uint8), eps, theta, nthreads, rows)
+    return rows
 
 
 def far_coeffs(M, r, eps2, t=None):
