@@ -393,6 +393,18 @@ def extensions(args, cmhip):
         pe[name] = {"ms": 1e3 * (time.perf_counter() - t0) / r, "potential": e["potential"]}
     pe["barnes_hut"]["rel_err"] = abs(pe["barnes_hut"]["potential"] - pe["exact"]["potential"]) \
         / abs(pe["exact"]["potential"])
+    # the snapshot's kinetic energy, momenta and centre of mass in one pass
+    # (57 bytes per body read), and the maximum acceleration (25 bytes per
+    # body); each call ends with a host read of the result
+    b.compute_forces_resident(eps)
+    for name, fn, per_body in (("energy_com", lambda: b.energy(eps, 0), 57),
+                               ("max_acceleration", b.max_acceleration, 25)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            fn()
+        t = (time.perf_counter() - t0) / 20
+        pe[name] = {"ms": 1e3 * t, "GB_per_s": per_body * nb / t / 1e9}
     b.close()
     res["potential_energy"] = pe
     log(json.dumps({"potential_energy": pe}))

@@ -48,7 +48,7 @@ ALL_FIELDS = F64_FIELDS + U8_FIELDS + I32_FIELDS
 EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_default_config", "cm_upload_soa", "cm_download_soa",
            "cm_get_count", "cm_compute_forces", "cm_verlet_step", "cm_energy",
-           "cm_potential_rows",
+           "cm_potential_rows", "cm_max_acceleration",
            "cm_direct_sample", "cm_get_force_stats", "cm_detect_collisions",
            "cm_resolve_merges", "cm_compact", "cm_generate_plummer",
            "cm_reset_force_stats", "cm_generate_disk", "cm_ensemble_create",
@@ -136,7 +136,7 @@ class EnergyReport(C.Structure):
     _fields_ = [("kinetic", C.c_double), ("potential", C.c_double),
                 ("momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
                 ("angular_momentum_mag", C.c_double), ("total_mass", C.c_double),
-                ("active_count", C.c_int32)]
+                ("active_count", C.c_int32), ("center_of_mass", C.c_double * 3)]
 
 
 class EnsembleStats(C.Structure):
@@ -179,6 +179,7 @@ def load_library():
         "cm_verlet_step": ([vp, f64, f64, i32], i32),
         "cm_energy": ([vp, f64, i32, C.POINTER(EnergyReport)], i32),
         "cm_potential_rows": ([vp, f64, i32, C.POINTER(C.c_double)], i32),
+        "cm_max_acceleration": ([vp, C.POINTER(C.c_double)], i32),
         "cm_direct_sample": ([vp, f64, vp, i32, vp, vp, vp], i32),
         "cm_get_force_stats": ([vp, C.POINTER(ForceStats)], i32),
         "cm_reset_force_stats": ([vp], i32),
@@ -489,7 +490,15 @@ class HipPhysicsBackend:
         return {"kinetic": r.kinetic, "potential": r.potential,
                 "momentum": tuple(r.momentum), "angular_momentum": tuple(r.angular_momentum),
                 "angular_momentum_mag": r.angular_momentum_mag,
-                "total_mass": r.total_mass, "active_count": r.active_count}
+                "total_mass": r.total_mass, "active_count": r.active_count,
+                "center_of_mass": tuple(r.center_of_mass)}
+
+    def max_acceleration(self):
+        """Largest |a| over active bodies of the accelerations the engine holds
+        (Engine::compute_max_acceleration); 0 when none is active."""
+        out = C.c_double(0.0)
+        self._check(self._L.cm_max_acceleration(self._h, C.byref(out)), "cm_max_acceleration")
+        return out.value
 
     def potential_rows(self, softening, method=2):
         """Per-body shares 0.5 m_i phi_i of the potential energy (method 1 exact,

@@ -142,6 +142,7 @@ int launch_direct_sample(Engine& e, double softening, const int32_t* targets,
 int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out);
 int launch_bh_potential(Engine& e, double softening, double* rows);
 int potential_rows(Engine& e, double softening, int method, double* rows);
+int max_acceleration(Engine& e, double* out);
 int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne);
 int collide_resolve(Engine& e, const cm_collision_config& cfg, int32_t* merged);
 int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index);

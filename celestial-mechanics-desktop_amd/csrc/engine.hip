@@ -542,6 +542,14 @@ int32_t cm_potential_rows(void* h, double softening, int32_t method, double* row
     return cm::potential_rows(*e, softening, method, rows);
 }
 
+int32_t cm_max_acceleration(void* h, double* out) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!out) return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "null output");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    return cm::max_acceleration(*e, out);
+}
+
 int32_t cm_direct_sample(void* h, double softening, const int32_t* targets,
                          int32_t n_targets, double* ax, double* ay, double* az) {
     Engine* e = as_engine(h);

@@ -11,7 +11,14 @@
 // PE is formed per body as a full row over j != i (tiled through LDS) and
 // halved at the end, or (with_pe = 2) from each body's Barnes-Hut potential,
 // 0.5 m_i phi_i, as the engine's launch_gpu_potential_energy does
-// (engine/src/cuda/potential_energy_kernel.cu:134-171).
+// (engine/src/cuda/potential_energy_kernel.cu:134-171).  The same pass sums
+// m x, m y, m z for the centre of mass of the engine's GPU energy snapshot
+// (k_reduce_com, engine/src/cuda/gpu_reduction.cu:232-262, finalised as
+// gpu_reduce_com does at :481-517).  k_max_accel / k_max_final are
+// Engine::compute_max_acceleration (engine/src/sim/engine.cpp:468-500,
+// k_reduce_max_accel at gpu_reduction.cu:269-283): the largest |a| over active
+// bodies, 0 when there are none.  A maximum does not depend on the order in
+// which it is taken, so that result is exact.
 #include <hip/hip_runtime.h>
 
 #include "cm_internal.h"
@@ -19,7 +26,7 @@
 namespace cm {
 
 constexpr int kRedBlocks = 512;
-constexpr int kQ = 10;   // ke, px, py, pz, lx, ly, lz, mass, pe, count
+constexpr int kQ = 13;   // ke, px, py, pz, lx, ly, lz, mass, pe, count, mx, my, mz
 
 struct Comp {
     double s, c;
@@ -83,6 +90,7 @@ k_energy_bodies(int n, const double* __restrict__ px, const double* __restrict__
         comp_add(v[7], mi);
         if (pe_rows) comp_add(v[8], pe_rows[i]);
         v[9].s += 1.0;
+        comp_add(v[10], mi * rx); comp_add(v[11], mi * ry); comp_add(v[12], mi * rz);
     }
     block_reduce_store(v, out);
 }
@@ -213,6 +221,56 @@ int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* ou
     out->total_mass = h[7];
     out->potential = with_pe ? h[8] : 0.0;
     out->active_count = (int32_t)h[9];
+    for (int k = 0; k < 3; k++) out->center_of_mass[k] = h[7] > 0.0 ? h[10 + k] / h[7] : 0.0;
+    return CM_OK;
+}
+
+__global__ void __launch_bounds__(256)
+k_max_accel(int n, const double* __restrict__ ax, const double* __restrict__ ay,
+            const double* __restrict__ az, const uint8_t* __restrict__ act,
+            double* __restrict__ block_max) {
+    __shared__ double sm[256];
+    double best = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        if (!act[i]) continue;
+        double x = ax[i], y = ay[i], z = az[i];
+        best = fmax(best, sqrt(x * x + y * y + z * z));
+    }
+    sm[threadIdx.x] = best;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_max[blockIdx.x] = sm[0];
+}
+
+__global__ void __launch_bounds__(256)
+k_max_final(int nblocks, double* __restrict__ block_max) {
+    __shared__ double sm[256];
+    double best = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) best = fmax(best, block_max[b]);
+    sm[threadIdx.x] = best;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) block_max[nblocks] = sm[0];
+}
+
+int max_acceleration(Engine& e, double* out) {
+    const int n = e.count;
+    if (n == 0) {
+        *out = 0.0;
+        return CM_OK;
+    }
+    hipLaunchKernelGGL(k_max_accel, dim3(kRedBlocks), dim3(256), 0, e.stream, n, e.f[AX],
+                       e.f[AY], e.f[AZ], e.act, e.red);
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, e.stream, kRedBlocks, e.red);
+    CM_HIP_TRY(e, hipGetLastError());
+    CM_HIP_TRY(e, hipMemcpyAsync(out, e.red + kRedBlocks, 8, hipMemcpyDeviceToHost, e.stream));
+    CM_HIP_TRY(e, hipStreamSynchronize(e.stream));
     return CM_OK;
 }
 

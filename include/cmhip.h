@@ -127,6 +127,11 @@ typedef struct {
     double angular_momentum_mag;
     double total_mass;
     int32_t active_count;
+    /* sum m r / sum m over active bodies, 0 when the total mass is not
+     * positive: the com fields of the engine's GPU energy snapshot
+     * (GpuTreeSolver::compute_gpu_energy, engine/src/cuda/gpu_tree.cu:1296-1300,
+     * gpu_reduce_com at engine/src/cuda/gpu_reduction.cu:481-517) */
+    double center_of_mass[3];
 } cm_energy_report;
 
 /* CollisionEvent (Collisions/CollisionEvent.cs:3-11). */
@@ -186,6 +191,11 @@ int32_t cm_energy(void* engine, double softening, int32_t with_potential,
  * compute_potential_bh_kernel writes (potential_energy_kernel.cu:29-111),
  * halved. */
 int32_t cm_potential_rows(void* engine, double softening, int32_t method, double* rows);
+/* Engine::compute_max_acceleration (engine/src/sim/engine.cpp:468-500,
+ * k_reduce_max_accel at engine/src/cuda/gpu_reduction.cu:269-283): the largest
+ * |a| = sqrt(ax^2 + ay^2 + az^2) over active bodies of the accelerations the
+ * engine holds (0 when none is active), the input of the adaptive time step. */
+int32_t cm_max_acceleration(void* engine, double* out);
 /* FP64 direct-sum accelerations on a subset of target bodies: the judge of
  * ForceErrorAnalyzer.CompareForces (Validation/ForceErrorAnalyzer.cs:67-92)
  * restricted to a sample.  Results are host arrays of length n_targets. */

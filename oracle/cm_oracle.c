@@ -1305,6 +1305,47 @@ void cmo_energy_exact(int n, const double* px, const double* py,
     out->l_mag = (double)sqrtl(lx * lx + ly * ly + lz * lz);
 }
 
+/* Centre of mass and total mass of the active bodies, as the engine's
+ * gpu_reduce_com finalises them (engine/src/cuda/gpu_reduction.cu:481-517,
+ * per-body terms m x, m y, m z, m at :232-262): sum m r / sum m, or 0 when the
+ * total mass is not positive.  Each sum is rounded once to double, then the
+ * quotient.  out4 = (cx, cy, cz, total mass). */
+void cmo_center_of_mass(int n, const double* px, const double* py, const double* pz,
+                        const double* m, const uint8_t* act, double* out4) {
+    /* Neumaier-compensated long double sums: a plain long double sum of 1e6
+     * equal masses already drifts by ~1e-14 relative, more than the device's
+     * compensated double sums. */
+    long double s[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; i++) {
+        if (!act[i]) continue;
+        long double t[4] = {(long double)m[i] * px[i], (long double)m[i] * py[i],
+                            (long double)m[i] * pz[i], m[i]};
+        for (int k = 0; k < 4; k++) {
+            long double u = s[k] + t[k];
+            c[k] += fabsl(s[k]) >= fabsl(t[k]) ? (s[k] - u) + t[k] : (t[k] - u) + s[k];
+            s[k] = u;
+        }
+    }
+    double M = (double)(s[3] + c[3]);
+    out4[3] = M;
+    for (int k = 0; k < 3; k++) out4[k] = M > 0.0 ? (double)(s[k] + c[k]) / M : 0.0;
+}
+
+/* Engine::compute_max_acceleration's CPU path (engine/src/sim/engine.cpp:
+ * 490-499) with the GPU kernel's per-body magnitude (k_reduce_max_accel,
+ * engine/src/cuda/gpu_reduction.cu:269-283): max over active bodies of
+ * sqrt(ax*ax + ay*ay + az*az), 0 when none is active. */
+double cmo_max_acceleration(int n, const double* ax, const double* ay, const double* az,
+                            const uint8_t* act) {
+    double best = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (!act[i]) continue;
+        double a = sqrt(ax[i] * ax[i] + ay[i] * ay[i] + az[i] * az[i]);
+        if (a > best) best = a;
+    }
+    return best;
+}
+
 static int cmp_double(const void* a, const void* b) {
     double x = *(const double*)a, y = *(const double*)b;
     return (x > y) - (x < y);

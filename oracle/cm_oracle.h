@@ -116,6 +116,10 @@ void cmo_energy_exact(int n, const double* px, const double* py,
                       const double* pz, const double* vx, const double* vy,
                       const double* vz, const double* m, const uint8_t* act,
                       double eps, int with_pe, int nthreads, cmo_energy_t* out);
+void cmo_center_of_mass(int n, const double* px, const double* py, const double* pz,
+                        const double* m, const uint8_t* act, double* out4);
+double cmo_max_acceleration(int n, const double* ax, const double* ay, const double* az,
+                            const uint8_t* act);
 typedef struct {
     double mean, max, rms, median;
     int32_t count;

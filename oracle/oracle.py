@@ -112,6 +112,10 @@ def lib():
     L.cmo_energy_exact.argtypes = [I, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
                                    u8p, D, I, I, C.POINTER(Energy)]
     L.cmo_energy_exact.restype = V
+    L.cmo_center_of_mass.argtypes = [I, f64p, f64p, f64p, f64p, u8p, f64p]
+    L.cmo_center_of_mass.restype = V
+    L.cmo_max_acceleration.argtypes = [I, f64p, f64p, f64p, u8p]
+    L.cmo_max_acceleration.restype = D
     L.cmo_force_error.argtypes = [I, C.c_void_p, f64p, f64p, f64p, f64p, f64p,
                                   f64p, C.POINTER(ForceErr)]
     L.cmo_force_error.restype = V
@@ -352,6 +356,21 @@ def energy_exact(s, eps, with_pe=True, nthreads=0):
                            _c(s["active"], np.uint8), eps, int(with_pe), nthreads,
                            C.byref(e))
     return {k: getattr(e, k) for k, _ in Energy._fields_}
+
+
+def center_of_mass(s):
+    """(cx, cy, cz, total mass) of the active bodies (gpu_reduce_com)."""
+    n = len(s["px"])
+    out = np.zeros(4)
+    lib().cmo_center_of_mass(n, *(_c(s[k], np.float64) for k in ("px", "py", "pz", "mass")),
+                             _c(s["active"], np.uint8), out)
+    return tuple(float(x) for x in out)
+
+
+def max_acceleration(acc, active):
+    """Largest |a| over active bodies (Engine::compute_max_acceleration)."""
+    n = len(acc[0])
+    return lib().cmo_max_acceleration(n, *(_c(a, np.float64) for a in acc), _c(active, np.uint8))
 
 
 def force_error(exact, test, active=None):
