@@ -51,6 +51,7 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_potential_rows", "cm_max_acceleration",
            "cm_direct_sample", "cm_get_force_stats", "cm_detect_collisions",
            "cm_resolve_merges", "cm_compact", "cm_generate_plummer",
+           "cm_generate_plummer_dotnet",
            "cm_reset_force_stats", "cm_generate_disk", "cm_ensemble_create",
            "cm_ensemble_destroy", "cm_ensemble_last_error", "cm_ensemble_configure",
            "cm_ensemble_upload", "cm_ensemble_download", "cm_ensemble_step",
@@ -188,6 +189,7 @@ def load_library():
         "cm_compact": ([vp, C.POINTER(i32), vp], i32),
         "cm_generate_plummer": ([i32, f64, f64, C.c_uint64, f64, i32,
                                  C.POINTER(SoAPtrs)], i32),
+        "cm_generate_plummer_dotnet": ([i32, f64, f64, i32, C.POINTER(SoAPtrs)], i32),
         "cm_generate_disk": ([i32, C.c_uint64, f64, f64, f64, f64, f64, f64, f64, f64,
                               C.POINTER(SoAPtrs)], i32),
         "cm_ensemble_create": ([i32, vp, i32, C.POINTER(vp)], i32),
@@ -288,6 +290,22 @@ def generate_plummer(count, total_mass=1.0, scale_length=1.0, seed=42,
                                             int(com_frame), C.byref(soa.ptrs()))
     if rc != CM_OK:
         raise CmError(rc, "cm_generate_plummer")
+    soa.count = count
+    soa.is_active[:count] = 1
+    return soa
+
+
+def generate_plummer_dotnet(count, total_mass=100.0, scale_length=2.0, seed=42,
+                            capacity=None):
+    """PlummerSphere.Generate(count, totalMass, scaleLength, seed)
+    (ScenarioGenerators/PlummerSphere.cs:41-116), with the reference's
+    defaults, body for body: the NextDouble stream of .NET's System.Random(seed).
+    All bodies active, radius 0.02; no centre-of-mass shift."""
+    soa = BodySoA(capacity or max(count, 1))
+    rc = load_library().cm_generate_plummer_dotnet(count, total_mass, scale_length, int(seed),
+                                                   C.byref(soa.ptrs()))
+    if rc != CM_OK:
+        raise CmError(rc, "cm_generate_plummer_dotnet")
     soa.count = count
     soa.is_active[:count] = 1
     return soa

@@ -3,6 +3,11 @@
 // Random numbers come from one SplitMix64 stream per call (the mixing
 // constants of the reference's DeterministicMode::deterministic_hash,
 // engine/src/sim/deterministic.cpp:21-28); NextDouble = top 53 bits / 2^53.
+// cm_generate_plummer_dotnet instead draws from the stream of .NET's seeded
+// System.Random (Knuth's subtractive generator as .NET implements it for an
+// explicit seed), so that it reproduces PlummerSphere.Generate(count, M, a,
+// seed) body for body.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -23,29 +28,57 @@ struct SplitMix64 {
     double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
 };
 
+// System.Random(seed) of .NET (the seeded, framework-compatible algorithm):
+// a 55-word lagged subtractive generator modulo 2^31 - 1, seeded from
+// 161803398 - |seed| with the 21-step index walk and four mixing rounds;
+// NextDouble = Sample() = draw * (1 / (2^31 - 1)).
+struct DotNetRandom {
+    int32_t a[56];
+    int inext = 0, inextp = 21;
+    explicit DotNetRandom(int32_t seed) {
+        const int32_t big = 2147483647;
+        int32_t mj = 161803398 - (seed == INT32_MIN ? big : (seed < 0 ? -seed : seed));
+        int32_t mk = 1;
+        a[0] = 0;
+        a[55] = mj;
+        for (int i = 1; i < 55; i++) {
+            int ii = (21 * i) % 55;
+            a[ii] = mk;
+            mk = mj - mk;
+            if (mk < 0) mk += big;
+            mj = a[ii];
+        }
+        for (int round = 1; round < 5; round++)
+            for (int i = 1; i < 56; i++) {
+                a[i] -= a[1 + (i + 30) % 55];
+                if (a[i] < 0) a[i] += big;
+            }
+    }
+    int32_t next() {
+        int p = inext + 1 >= 56 ? 1 : inext + 1;
+        int q = inextp + 1 >= 56 ? 1 : inextp + 1;
+        int32_t v = a[p] - a[q];
+        if (v == 2147483647) v--;
+        if (v < 0) v += 2147483647;
+        a[p] = v;
+        inext = p;
+        inextp = q;
+        return v;
+    }
+    double uniform() { return next() * (1.0 / 2147483647.0); }
+};
+
 constexpr double kPi = 3.14159265358979323846;
 
-}  // namespace
-
-extern "C" int32_t cm_generate_plummer(int32_t n, double total_mass,
-                                       double scale_length, uint64_t seed,
-                                       double mass_fraction_cut,
-                                       int32_t com_frame,
-                                       const cm_soa_ptrs* out) {
-    if (n < 0 || !out || !out->pos_x || !out->pos_y || !out->pos_z ||
-        !out->vel_x || !out->vel_y || !out->vel_z || !out->mass)
-        return CM_ERR_INVALID_ARGUMENT;
-    if (!(total_mass > 0.0) || !(scale_length > 0.0) ||
-        !(mass_fraction_cut > 0.0) || mass_fraction_cut > 1.0)
-        return CM_ERR_INVALID_ARGUMENT;
-    if (n == 0) return CM_OK;
-    SplitMix64 rng(seed);
+// PlummerSphere.Generate's per-body loop (ScenarioGenerators/PlummerSphere.cs:
+// 52-106), drawing NextDouble from `rng` in the reference's order.
+template <class Rng>
+void plummer_bodies(Rng& rng, int32_t n, double total_mass, double a, double cut,
+                    const cm_soa_ptrs* out) {
     const double body_mass = total_mass / n;
-    const double a = scale_length;
-    // PlummerSphere.Generate (ScenarioGenerators/PlummerSphere.cs:41-116)
     for (int32_t i = 0; i < n; i++) {
         double u = rng.uniform();
-        if (mass_fraction_cut < 1.0) u *= mass_fraction_cut;   // truncation
+        if (cut < 1.0) u *= cut;   // truncation
         u = std::max(u, 1e-10);
         u = std::min(u, 1.0 - 1e-10);
         double r = a / std::sqrt(std::pow(u, -2.0 / 3.0) - 1.0);
@@ -71,6 +104,26 @@ extern "C" int32_t cm_generate_plummer(int32_t n, double total_mass,
         out->mass[i] = body_mass;
         if (out->radius) out->radius[i] = 0.02;
     }
+}
+
+bool plummer_args_ok(int32_t n, double total_mass, double scale_length, const cm_soa_ptrs* out) {
+    return n >= 0 && out && out->pos_x && out->pos_y && out->pos_z && out->vel_x &&
+           out->vel_y && out->vel_z && out->mass && total_mass > 0.0 && scale_length > 0.0;
+}
+
+}  // namespace
+
+extern "C" int32_t cm_generate_plummer(int32_t n, double total_mass,
+                                       double scale_length, uint64_t seed,
+                                       double mass_fraction_cut,
+                                       int32_t com_frame,
+                                       const cm_soa_ptrs* out) {
+    if (!plummer_args_ok(n, total_mass, scale_length, out) ||
+        !(mass_fraction_cut > 0.0) || mass_fraction_cut > 1.0)
+        return CM_ERR_INVALID_ARGUMENT;
+    if (n == 0) return CM_OK;
+    SplitMix64 rng(seed);
+    plummer_bodies(rng, n, total_mass, scale_length, mass_fraction_cut, out);
     if (com_frame) {
         long double m = 0, cx = 0, cy = 0, cz = 0, ux = 0, uy = 0, uz = 0;
         for (int32_t i = 0; i < n; i++) {
@@ -86,6 +139,17 @@ extern "C" int32_t cm_generate_plummer(int32_t n, double total_mass,
             out->vel_x[i] -= vx; out->vel_y[i] -= vy; out->vel_z[i] -= vz;
         }
     }
+    return CM_OK;
+}
+
+extern "C" int32_t cm_generate_plummer_dotnet(int32_t n, double total_mass,
+                                              double scale_length, int32_t seed,
+                                              const cm_soa_ptrs* out) {
+    if (!plummer_args_ok(n, total_mass, scale_length, out) || seed < 0)
+        return CM_ERR_INVALID_ARGUMENT;
+    if (n == 0) return CM_OK;
+    DotNetRandom rng(seed);
+    plummer_bodies(rng, n, total_mass, scale_length, 1.0, out);
     return CM_OK;
 }
 

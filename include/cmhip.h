@@ -435,6 +435,13 @@ int32_t cm_generate_disk(int32_t n, uint64_t seed, double star_radius,
 int32_t cm_generate_plummer(int32_t n, double total_mass, double scale_length,
                             uint64_t seed, double mass_fraction_cut,
                             int32_t com_frame, const cm_soa_ptrs* out);
+/* PlummerSphere.Generate(count, totalMass, scaleLength, seed)
+ * (ScenarioGenerators/PlummerSphere.cs:41-116) body for body: the same
+ * NextDouble stream as .NET's System.Random(seed), no truncation, no
+ * centre-of-mass shift.  seed >= 0 (the reference seeds a negative seed from
+ * the clock, which no caller can reproduce).  Writes pos/vel/mass/radius. */
+int32_t cm_generate_plummer_dotnet(int32_t n, double total_mass, double scale_length,
+                                   int32_t seed, const cm_soa_ptrs* out);
 
 #ifdef __cplusplus
 }

@@ -174,6 +174,12 @@ int cmo_resolve_events(int ne, const cmo_event_t* ev, int n, double* px, double*
                        double mass_loss_cap, int promote_remnants, int max_per_frame,
                        int max_per_body, int32_t* merge_count_scratch);
 
+/* .NET System.Random(seed) draws and PlummerSphere.Generate (cm_oracle_ext.c) */
+void cmo_dotnet_random(int32_t seed, int count, int32_t* out);
+void cmo_plummer_dotnet(int n, double total_mass, double a, int32_t seed, double* px,
+                        double* py, double* pz, double* vx, double* vy, double* vz,
+                        double* m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include "cm_oracle.h"
@@ -435,4 +436,96 @@ int cmo_resolve_events(int ne, const cmo_event_t* ev, int n, double* px, double*
         resolved++;
     }
     return resolved;
+}
+
+/* ---- .NET System.Random and PlummerSphere.Generate ---------------------- */
+/* System.Random(seed) as .NET implements it for an explicit seed (Knuth's
+ * subtractive generator, "Seminumerical Algorithms" 3.2.2, in the form of
+ * Numerical Recipes' ran3): 55 words modulo MBIG = 2^31 - 1, seeded from
+ * MSEED - |seed| (MSEED = 161803398) by the 21-step index walk, four mixing
+ * rounds, then draws a[i] - a[i + 31 mod 55] with the two indices starting at
+ * 0 and 21.  NextDouble = draw * (1 / MBIG).  The reference's
+ * PlummerSphere.Generate (ScenarioGenerators/PlummerSphere.cs:44) draws from
+ * it; .NET itself is absent here, so known answers of Random(0) pin it. */
+typedef struct { int32_t a[56]; int p, q; } cmo_netrand;
+
+static void netrand_seed(cmo_netrand* r, int32_t seed) {
+    const int32_t mbig = 2147483647;
+    int32_t mj = 161803398 - (seed == INT32_MIN ? mbig : abs(seed));
+    int32_t mk = 1;
+    memset(r->a, 0, sizeof(r->a));
+    r->a[55] = mj;
+    for (int i = 1; i < 55; i++) {
+        int ii = (21 * i) % 55;
+        r->a[ii] = mk;
+        mk = mj - mk;
+        if (mk < 0) mk += mbig;
+        mj = r->a[ii];
+    }
+    for (int k = 1; k < 5; k++)
+        for (int i = 1; i < 56; i++) {
+            r->a[i] -= r->a[1 + (i + 30) % 55];
+            if (r->a[i] < 0) r->a[i] += mbig;
+        }
+    r->p = 0;
+    r->q = 21;
+}
+
+static int32_t netrand_next(cmo_netrand* r) {
+    if (++r->p >= 56) r->p = 1;
+    if (++r->q >= 56) r->q = 1;
+    int32_t v = r->a[r->p] - r->a[r->q];
+    if (v == 2147483647) v--;
+    if (v < 0) v += 2147483647;
+    r->a[r->p] = v;
+    return v;
+}
+
+static double netrand_double(cmo_netrand* r) { return netrand_next(r) * (1.0 / 2147483647.0); }
+
+/* The first `count` raw draws (InternalSample) of System.Random(seed). */
+void cmo_dotnet_random(int32_t seed, int count, int32_t* out) {
+    cmo_netrand r;
+    netrand_seed(&r, seed);
+    for (int i = 0; i < count; i++) out[i] = netrand_next(&r);
+}
+
+/* PlummerSphere.Generate(count, totalMass, scaleLength, seed)
+ * (ScenarioGenerators/PlummerSphere.cs:41-116), G = 1. */
+void cmo_plummer_dotnet(int n, double total_mass, double a, int32_t seed, double* px,
+                        double* py, double* pz, double* vx, double* vy, double* vz,
+                        double* m) {
+    cmo_netrand r;
+    netrand_seed(&r, seed);
+    const double pi = 3.14159265358979323846;
+    double bm = total_mass / n;
+    for (int i = 0; i < n; i++) {
+        double u = netrand_double(&r);
+        if (u < 1e-10) u = 1e-10;                 /* Math.Max(u, 1e-10) */
+        if (u > 1.0 - 1e-10) u = 1.0 - 1e-10;     /* Math.Min(u, 1 - 1e-10) */
+        double rad = a / sqrt(pow(u, -2.0 / 3.0) - 1.0);
+        double ct = 2.0 * netrand_double(&r) - 1.0;
+        double st = sqrt(1.0 - ct * ct);
+        double phi = 2.0 * pi * netrand_double(&r);
+        px[i] = rad * st * cos(phi);
+        py[i] = rad * st * sin(phi);
+        pz[i] = rad * ct;
+        double vesc = sqrt(2.0 * 1.0 * total_mass / sqrt(rad * rad + a * a));
+        double speed;
+        for (;;) {
+            double q = netrand_double(&r);
+            double g = q * q * pow(1.0 - q * q, 3.5);
+            if (netrand_double(&r) * 0.1 < g) {
+                speed = q * vesc;
+                break;
+            }
+        }
+        double vct = 2.0 * netrand_double(&r) - 1.0;
+        double vst = sqrt(1.0 - vct * vct);
+        double vphi = 2.0 * pi * netrand_double(&r);
+        vx[i] = speed * vst * cos(vphi);
+        vy[i] = speed * vst * sin(vphi);
+        vz[i] = speed * vct;
+        m[i] = bm;
+    }
 }

@@ -112,6 +112,10 @@ def lib():
     L.cmo_energy_exact.argtypes = [I, f64p, f64p, f64p, f64p, f64p, f64p, f64p,
                                    u8p, D, I, I, C.POINTER(Energy)]
     L.cmo_energy_exact.restype = V
+    L.cmo_dotnet_random.argtypes = [C.c_int32, I, i32p]
+    L.cmo_dotnet_random.restype = V
+    L.cmo_plummer_dotnet.argtypes = [I, D, D, C.c_int32] + [f64p] * 7
+    L.cmo_plummer_dotnet.restype = V
     L.cmo_center_of_mass.argtypes = [I, f64p, f64p, f64p, f64p, u8p, f64p]
     L.cmo_center_of_mass.restype = V
     L.cmo_max_acceleration.argtypes = [I, f64p, f64p, f64p, u8p]
@@ -371,6 +375,22 @@ def max_acceleration(acc, active):
     """Largest |a| over active bodies (Engine::compute_max_acceleration)."""
     n = len(acc[0])
     return lib().cmo_max_acceleration(n, *(_c(a, np.float64) for a in acc), _c(active, np.uint8))
+
+
+def dotnet_random(seed, count):
+    """The first `count` raw draws of .NET's System.Random(seed)."""
+    out = np.zeros(count, np.int32)
+    lib().cmo_dotnet_random(seed, count, out)
+    return out
+
+
+def plummer_dotnet(n, total_mass=100.0, scale_length=2.0, seed=42):
+    """PlummerSphere.Generate(n, totalMass, scaleLength, seed): dict of
+    px, py, pz, vx, vy, vz, mass."""
+    out = {k: np.zeros(n) for k in ("px", "py", "pz", "vx", "vy", "vz", "mass")}
+    lib().cmo_plummer_dotnet(n, total_mass, scale_length, seed,
+                             *(out[k] for k in ("px", "py", "pz", "vx", "vy", "vz", "mass")))
+    return out
 
 
 def force_error(exact, test, active=None):
