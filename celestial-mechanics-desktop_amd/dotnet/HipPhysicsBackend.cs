@@ -93,7 +93,27 @@ internal static unsafe class CmHip
     [DllImport(Lib)] public static extern int cm_set_softening(IntPtr engine, ref SofteningConfig cfg);
     [DllImport(Lib)] public static extern int cm_default_collision_config(ref CollisionConfig cfg);
     [DllImport(Lib)] public static extern int cm_configure_collisions(IntPtr engine, ref CollisionConfig cfg);
+    // cm_energy_report: EnergyCalculator quantities plus the centre of mass of
+    // the engine's GPU energy snapshot.
+    [StructLayout(LayoutKind.Sequential)]
+    public struct EnergyReport
+    {
+        public double Kinetic;
+        public double Potential;
+        public fixed double Momentum[3];
+        public fixed double AngularMomentum[3];
+        public double AngularMomentumMag;
+        public double TotalMass;
+        public int ActiveCount;
+        public fixed double CenterOfMass[3];
+    }
+
     [DllImport(Lib)] public static extern int cm_resolve_collisions(IntPtr engine, out int nResolved);
+    // withPotential: 0 none, 1 exact pair sum, 2 Barnes-Hut potential
+    [DllImport(Lib)] public static extern int cm_energy(IntPtr engine, double softening, int withPotential, out EnergyReport report);
+    [DllImport(Lib)] public static extern int cm_max_acceleration(IntPtr engine, out double maxAccel);
+    // PlummerSphere.Generate(count, totalMass, scaleLength, seed) body for body
+    [DllImport(Lib)] public static extern int cm_generate_plummer_dotnet(int n, double totalMass, double scaleLength, int seed, ref SoAPtrs output);
 
     public static void Check(IntPtr engine, int rc, string what)
     {
