@@ -28,6 +28,8 @@ LIB_PATH = os.path.join(_PKG_DIR, "libcmhip.so")
 
 # ---- constants of include/cmhip.h ----------------------------------------
 CM_OK = 0
+CM_ERR_HIP, CM_ERR_OUT_OF_MEMORY, CM_ERR_INVALID_ARGUMENT = -1, -2, -3
+CM_ERR_POOL_EXHAUSTED, CM_ERR_DEVICE_NOT_FOUND, CM_ERR_NOT_INITIALIZED = -4, -5, -6
 ERRORS = {-1: "HipError", -2: "OutOfMemory", -3: "InvalidArgument",
           -4: "PoolExhausted", -5: "DeviceNotFound", -6: "NotInitialized"}
 BACKEND_DIRECT_PAIRWISE = 0

@@ -19,6 +19,11 @@ constexpr int kStackCap = 1536;         // per-wave LDS traversal stack (ints)
 constexpr int kSortTile = 2048;         // keys per radix-sort tile
 constexpr int kSortThreads = 256;
 constexpr int kTreeTile = 1024;         // bodies per tree-build tile
+// FP32 pair loops may leave a body's own (dx = 0) term in the sum when the
+// softening makes it exactly zero: r^2 = eps^2, rsq(eps^2)^3 <= 1e18 for
+// eps^2 >= 1e-12, so m rsq^3 stays finite for any mass below 3e20 and the
+// term is +-0.  Below this softening the loops skip the body itself by index.
+constexpr double kSelfTermMinEps2 = 1e-12;
 
 // rows of per-tile node counts the tree build keeps per level
 inline int tree_tiles(int capacity) { return (capacity + kTreeTile - 1) / kTreeTile + 1; }
@@ -45,6 +50,8 @@ struct TreeHeader {
     unsigned long long pot_int[5];   // terms counted by potential evaluations (unused)
     int32_t n_groups;             // target groups formed for the traversal
     uint32_t work_next[8];        // per-XCD queues: next group of each eighth
+    int32_t overflow_run;         // set by any evaluation of a run whose tree outgrew
+                                  // the node pool; cleared by the host per API call
 };
 
 // Workgroups of one launch are dispatched round-robin over the 8 XCDs of an
@@ -162,6 +169,7 @@ void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
 
 int set_error(Engine& e, int code, const std::string& msg);
 int finalize_stats(Engine& e);   // syncs the stream, fills e.stats
+int begin_run(Engine& e);        // clears the run's sticky overflow flag
 
 }  // namespace cm
 

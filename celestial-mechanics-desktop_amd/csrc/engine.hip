@@ -36,7 +36,9 @@ int finalize_stats(Engine& e) {
         s.n_active = h.n_active;
         s.n_nodes = h.n_nodes;
         s.n_groups = h.n_groups;
-        s.flags = h.overflow ? 1 : 0;
+        // any evaluation of the run (not only the last, whose header this
+        // is) that outgrew the node pool fails the call
+        s.flags = (h.overflow || h.overflow_run) ? 1 : 0;
         if (e.timing) {
             s.ms_tree = ms(0, 3);
             s.ms_sort = ms(1, 2);
@@ -73,6 +75,15 @@ int finalize_stats(Engine& e) {
 __global__ void k_reset_acc(TreeHeader* hdr) {
     hdr->acc_int = 0ull;
     hdr->acc_far = 0ull;
+}
+
+__global__ void k_begin_run(TreeHeader* hdr) { hdr->overflow_run = 0; }
+
+int begin_run(Engine& e) {
+    if (e.cfg.kind != CM_BACKEND_BARNES_HUT || e.count == 0) return CM_OK;
+    hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, e.stream, e.hdr);
+    CM_HIP_TRY(e, hipGetLastError());
+    return CM_OK;
 }
 
 // ---- Verlet kernels --------------------------------------------------------
@@ -441,7 +452,8 @@ int32_t cm_compute_forces(void* h, double softening) {
     if (!(softening >= 0.0))
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "softening must be >= 0");
     CM_HIP_TRY(*e, hipSetDevice(e->device));
-    int rc = cm::compute_forces(*e, softening);
+    int rc = cm::begin_run(*e);
+    if (rc == CM_OK) rc = cm::compute_forces(*e, softening);
     if (rc != CM_OK) return rc;
     rc = cm::finalize_stats(*e);
     if (rc != CM_OK) return rc;
@@ -459,6 +471,7 @@ int32_t cm_verlet_step(void* h, double softening, double dt, int32_t n_steps) {
     constexpr int kChunk = 256;   // steps per host loop iteration
     double dts[kChunk];
     for (int k = 0; k < kChunk; k++) dts[k] = dt;
+    if (int rc0 = cm::begin_run(*e); rc0 != CM_OK) return rc0;
     for (int s = 0; s < n_steps; s += kChunk) {
         int rc = cm::verlet_run(*e, softening, dts, std::min(kChunk, n_steps - s));
         if (rc != CM_OK) return rc;
@@ -484,6 +497,7 @@ int32_t cm_yoshida4_step(void* h, double softening, double dt, int32_t n_steps) 
     constexpr int kChunk = 255;   // sub-steps per host loop iteration (85 steps)
     double dts[kChunk];
     for (int k = 0; k < kChunk; k++) dts[k] = sub[k % 3];
+    if (int rc0 = cm::begin_run(*e); rc0 != CM_OK) return rc0;
     for (int s = 0; s < n_steps; s += kChunk / 3) {
         int rc = cm::verlet_run(*e, softening, dts, 3 * std::min(kChunk / 3, n_steps - s));
         if (rc != CM_OK) return rc;
@@ -620,6 +634,7 @@ int32_t cm_nbody_step(void* h, double softening, double dt, int32_t n_steps,
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "bad softening, step count or flag");
     CM_HIP_TRY(*e, hipSetDevice(e->device));
     int32_t total = 0;
+    if (int rc0 = cm::begin_run(*e); rc0 != CM_OK) return rc0;
     for (int s = 0; s < n_steps; s++) {
         int rc = cm::verlet_run(*e, softening, &dt, 1);
         if (rc == CM_OK && collisions) {

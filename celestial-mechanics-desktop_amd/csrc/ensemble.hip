@@ -136,11 +136,11 @@ k_ens_run(EnsArgs a) {
             // phase 3: forces, then phases 4-5 (second half kick, rotate)
             if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                 // two targets per thread share every source read; inactive
-                // sources carry zero mass and, with eps2 > 0, the self term
-                // is exactly zero, so neither needs a branch (adding an exact
-                // zero leaves each float sum unchanged)
+                // sources carry zero mass and, with eps2 >= kSelfTermMinEps2,
+                // the self term is exactly zero, so neither needs a branch
+                // (adding an exact zero leaves each float sum unchanged)
                 const float e2 = (float)a.eps2;
-                const bool skip_self = !(a.eps2 > 0.0);
+                const bool skip_self = !(a.eps2 >= kSelfTermMinEps2);
                 const int T = (nb + 1) / 2;   // target slots of two targets each
                 if (T < NT) {
                     // fewer slots than threads: the idle threads take source

@@ -568,9 +568,9 @@ k_traverse(TravArgs a) {
                 if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
                     const float e2 = (float)a.eps2;
-                    if (cells || (!shell && e2 > 0.0f)) {
-                        // with eps2 > 0 the self term (dx = 0) is exactly zero;
-                        // entries past cnt are massless and far away
+                    if (cells || (!shell && a.eps2 >= kSelfTermMinEps2)) {
+                        // with this softening the self term (dx = 0) is
+                        // exactly zero; entries past cnt are massless and far
                         auto term = [&](const float4 v) {
                             float dx = txw - v.x, dy = tyw - v.y, dz = tzw - v.z;
                             float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));

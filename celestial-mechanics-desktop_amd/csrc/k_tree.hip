@@ -26,8 +26,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "cm_internal.h"
+#include "cm_testing.h"
 
 namespace cm {
 
@@ -428,6 +430,7 @@ k_tree_scan(int levels, int cap, int ntiles, int tstride, int node_cap,
         for (int l = levels + 1; l < kMaxLevels + 3; l++) hdr->level_start[l] = end;
         int over = running > node_cap ? 1 : 0;
         hdr->overflow = over;
+        if (over) hdr->overflow_run = 1;
         hdr->n_nodes = over ? node_cap : end;
         hdr->n_levels_used = used;
         int na = hdr->n_active;
@@ -776,3 +779,62 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
 }
 
 }  // namespace cm
+
+// ---- test-only export (csrc/cm_testing.h) ----------------------------------
+
+extern "C" int32_t cm_test_tree(void* h, int32_t node_cap, uint64_t* keys, int32_t* order,
+                                double* box4, int32_t* n_active, int32_t* n_nodes,
+                                int32_t* info, double* com_mass, int32_t* level_start) {
+    using namespace cm;
+    Engine* e = static_cast<Engine*>(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (node_cap < 0) return set_error(*e, CM_ERR_INVALID_ARGUMENT, "negative node_cap");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    const int n = e->count;
+    if (n == 0) {
+        if (n_active) *n_active = 0;
+        if (n_nodes) *n_nodes = 0;
+        return CM_OK;
+    }
+    const uint64_t* k = nullptr;
+    const int32_t* o = nullptr;
+    int rc = build_tree(*e, &k, &o);
+    if (rc != CM_OK) return rc;
+    TreeHeader hh;
+    CM_HIP_TRY(*e, hipMemcpyAsync(&hh, e->hdr, sizeof(hh), hipMemcpyDeviceToHost, e->stream));
+    CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
+    if (hh.overflow) return set_error(*e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");
+    if (n_active) *n_active = hh.n_active;
+    if (n_nodes) *n_nodes = hh.n_nodes;
+    if (box4) for (int q = 0; q < 4; q++) box4[q] = hh.box[q];
+    if (level_start) for (int q = 0; q < kMaxLevels + 3; q++) level_start[q] = hh.level_start[q];
+    if (keys)
+        CM_HIP_TRY(*e, hipMemcpyAsync(keys, k, (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
+    if (order)
+        CM_HIP_TRY(*e, hipMemcpyAsync(order, o, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
+    const int nn = std::min(node_cap, hh.n_nodes);
+    std::vector<int4> inf((size_t)std::max(nn, 1));
+    std::vector<double4> cm4((size_t)std::max(nn, 1));
+    if (nn > 0) {
+        CM_HIP_TRY(*e, hipMemcpyAsync(inf.data(), e->nd_info, (size_t)nn * sizeof(int4),
+                                      hipMemcpyDeviceToHost, e->stream));
+        CM_HIP_TRY(*e, hipMemcpyAsync(cm4.data(), e->nd_cm, (size_t)nn * sizeof(double4),
+                                      hipMemcpyDeviceToHost, e->stream));
+    }
+    CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
+    for (int q = 0; q < nn; q++) {
+        if (info) {
+            int32_t* r = info + 5 * (size_t)q;
+            r[0] = inf[q].x;
+            r[1] = inf[q].y & 255;
+            r[2] = inf[q].y >> 8;
+            r[3] = inf[q].z;
+            r[4] = inf[q].w;
+        }
+        if (com_mass) {
+            double* r = com_mass + 4 * (size_t)q;
+            r[0] = cm4[q].x; r[1] = cm4[q].y; r[2] = cm4[q].z; r[3] = cm4[q].w;
+        }
+    }
+    return CM_OK;
+}

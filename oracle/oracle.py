@@ -257,9 +257,9 @@ def stable_sort_keys(keys):
     return ks, order
 
 
-def tree_build(s, levels, leaf_cap):
+def tree_build(s, levels, leaf_cap, node_cap=None):
     n = len(s["px"])
-    cap = 2 * n + 16
+    cap = node_cap or 2 * n + 16
     box = np.zeros(4)
     na = C.c_int32(0)
     ks = np.zeros(n, dtype=np.uint64)

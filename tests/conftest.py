@@ -46,3 +46,23 @@ def plummer():
         return cmhip.generate_plummer(n, 1.0, 1.0, seed=seed, mass_fraction_cut=cut,
                                       com_frame=com)
     return make
+
+
+def pair_collapse_state(n_pairs=10000, v=50.0, dt=1e-3, delta=1e-9, seed=31):
+    """Pairs of light bodies that approach each other at speed v: after one
+    drift of dt their separation is delta (coincident at 21-bit key
+    resolution, so each pair hangs a long single-child chain off the octree),
+    after the second drift they have passed each other and are 2 v dt - delta
+    apart.  Masses are 1e-12, so the forces change neither drift measurably."""
+    import cmhip
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1.0, 1.0, (n_pairs, 3))
+    u = rng.normal(size=(n_pairs, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    d0 = 2 * v * dt + delta
+    p = np.concatenate([c - 0.5 * d0 * u, c + 0.5 * d0 * u])
+    w = np.concatenate([v * u, -v * u])
+    n = 2 * n_pairs
+    return cmhip.BodySoA.from_arrays(pos_x=p[:, 0], pos_y=p[:, 1], pos_z=p[:, 2],
+                                     vel_x=w[:, 0], vel_y=w[:, 1], vel_z=w[:, 2],
+                                     mass=np.full(n, 1e-12), is_active=np.ones(n))

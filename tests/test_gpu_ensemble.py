@@ -75,3 +75,31 @@ def test_ensemble_energy_matches_oracle():
         st = system_state(soa, off[s], off[s + 1])
         x = O.energy_exact(st, EPS, True)
         assert abs(e[s] - (x["ke"] + x["pe"])) <= 1e-13 * abs(x["ke"] + x["pe"])
+
+
+@pytest.mark.parametrize("threads", [256, 512, 1024])
+def test_ensemble_fp64_bit_exact_up_to_2048_bodies(threads):
+    """The ensemble configuration's full size range, N in [8, 2048]: systems of
+    513-2048 bodies (more bodies than threads, several targets per thread) and
+    the smallest ones, bit for bit against the CPU restatement, for every
+    workgroup size the autotuner can pick."""
+    soa, off = cmhip.generate_ensemble(40, seed=12, n_min=8, n_max=2048)
+    nb = np.diff(off)
+    print(f"ensemble: threads={threads} sizes {nb.min()}..{nb.max()}", flush=True)
+    assert nb.max() > 1900 and nb.min() < 100
+    ens = cmhip.HipEnsemble(off, precision=cmhip.PREC_FP64, threads=threads, steps_per_launch=3)
+    ens.upload(soa)
+    ens.step(EPS, 1e-3, 5)          # one graph replay + two plain launches
+    out = cmhip.BodySoA(soa.count)
+    ens.download(out)
+    picks = sorted({int(np.argmax(nb)), int(np.argmin(nb)),
+                    int(np.argsort(nb)[len(nb) // 2]),
+                    int(np.where((nb > 512) & (nb < 1100))[0][0])})
+    for s in picks:
+        st = system_state(soa, off[s], off[s + 1])
+        for _ in range(5):
+            O.verlet_step(st, 1e-3, lambda x: O.direct_gk(x, EPS))
+        b0, b1 = off[s], off[s + 1]
+        np.testing.assert_array_equal(out.pos_x[b0:b1], st["px"])
+        np.testing.assert_array_equal(out.vel_z[b0:b1], st["vz"])
+        np.testing.assert_array_equal(out.old_acc_y[b0:b1], st["oay"])

@@ -83,3 +83,39 @@ def test_plummer_energy_drift_10k_fp32_vs_fp64():
     assert drift["fp32"] <= 2.0 * drift["fp64"] + 1e-6
     assert drift["fp32_far"] <= 2.0 * drift["fp64"] + 1e-6
     assert drift["fp64"] < 1e-3
+
+
+@pytest.mark.parametrize("name,prec,leaf,far", [
+    ("fp64", cmhip.PREC_FP64, 16, 0.0),
+    ("fp32_split", cmhip.PREC_FP32_SPLIT, 8, 0.0),
+    ("fp32_split_far", cmhip.PREC_FP32_SPLIT, 8, 0.15),
+])
+def test_energy_drift_no_worse_than_reference_path(name, prec, leaf, far):
+    """10k-step energy drift judged against the reference's own path, after the
+    pattern of SoAStabilityTests.cs:177-218 (drift compared with a baseline
+    run): tests/golden/drift_bh_csharp_plummer2048.json holds |dE/E| of the
+    oracle's restatement of SoAVerletIntegrator.Step + BarnesHutBackend
+    (serial insertion octree, theta 0.5) every 1000 steps on the same state
+    (N = 2048 Plummer, seed 42, softening 1e-2, dt = 1e-3), written by
+    tests/golden/make_drift.py.  Each GPU path's largest |dE/E| over the same
+    checkpoints must not exceed the reference's (slack 1.0: no worse)."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden",
+                           "drift_bh_csharp_plummer2048.json")) as fh:
+        ref = json.load(fh)
+    n, eps, dt = ref["n"], ref["softening"], ref["dt"]
+    soa = cmhip.generate_plummer(n, 1.0, 1.0, seed=ref["seed"], mass_fraction_cut=0.999,
+                                 com_frame=True)
+    be = cmhip.HipPhysicsBackend(n, precision=prec, theta=ref["theta"], leaf_capacity=leaf,
+                                 far_field_ratio=far)
+    be.upload(soa)
+    e0 = _energy(be, eps)
+    assert abs(e0 - ref["e0"]) <= 1e-13 * abs(ref["e0"])   # same initial state
+    worst = 0.0
+    for k in range(10):
+        be.step(eps, dt, 1000)
+        worst = max(worst, abs((_energy(be, eps) - e0) / e0))
+    print(f"{name}: max |dE/E| over 10k steps {worst:.3e}, reference path "
+          f"{ref['drift_max_abs']:.3e}", flush=True)
+    assert worst <= 1.0 * ref["drift_max_abs"], (worst, ref["drift_max_abs"])

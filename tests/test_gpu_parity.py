@@ -262,3 +262,42 @@ def test_barnes_hut_oversized_leaves_bit_exact():
     ref, _ = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=16, group_size=64)
     _eq(_acc(soa), ref)
     assert (np.abs(soa.acc_x[:len(p)]) > 0).all()
+
+
+def test_pool_overflow_in_an_earlier_step_is_reported():
+    """A run whose first force evaluation outgrows the node pool fails with
+    CM_ERR_POOL_EXHAUSTED although its last evaluation fits (the state's node
+    counts are pinned on the CPU by
+    test_pair_collapse_state_overflows_only_after_the_first_drift); the next
+    run, whose trees all fit, succeeds."""
+    from conftest import pair_collapse_state
+    soa = pair_collapse_state()
+    n = soa.count
+    print(f"sticky overflow: n={n}", flush=True)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5, morton_levels=21,
+                                 leaf_capacity=1, group_size=64)
+    be.upload(soa)
+    with pytest.raises(cmhip.CmError) as err:
+        be.step(EPS, 1e-3, 2)
+    assert err.value.code == cmhip.CM_ERR_POOL_EXHAUSTED
+    be.step(EPS, 1e-3, 1)
+    assert be.stats()["flags"] == 0
+
+
+@pytest.mark.parametrize("eps", [1e-14, 0.0])
+def test_fp32_split_tiny_softening_is_finite(plummer, eps):
+    """Softening below 1e-6 (eps^2 < 1e-12) takes the self-skipping loop in
+    FP32 split, where rsq(eps^2)^3 would overflow for the body's own term:
+    the result stays finite and within the FP32 tolerance of the FP64 path."""
+    n = 20000
+    soa = plummer(n, cut=0.999)
+    a = soa.copy()
+    b = soa.copy()
+    for work, prec in ((a, cmhip.PREC_FP64), (b, cmhip.PREC_FP32_SPLIT)):
+        be = cmhip.HipPhysicsBackend(n, precision=prec, theta=0.5, leaf_capacity=8)
+        be.compute_forces(work, eps)
+        be.close()
+    got, ref = _acc(b), _acc(a)
+    assert all(np.isfinite(x).all() for x in got)
+    r = _rel(got, ref)
+    assert np.median(r) <= 2e-6 and r.max() <= 1e-3, (np.median(r), r.max())

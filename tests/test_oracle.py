@@ -113,6 +113,33 @@ def test_direct_sums_match_extended_precision_brute_force(fn):
     assert all((np.asarray(g)[~act] == 0).all() for g in got)
 
 
+def test_direct_targets_is_the_sampled_direct_sum():
+    """The force judge (ForceErrorAnalyzer's exact forces, ForceErrorAnalyzer.cs:
+    67-92, on a sample of targets): its rows equal the pinned full sweep
+    (CpuParallelBackend order) at the sampled indices bit for bit, with
+    inactive targets and sources, repeated and unsorted targets; and they
+    agree with the long-double brute force.  A dropped source, a wrong index
+    mapping or a self term fails one of these."""
+    s = random_state(300, seed=11)
+    s["active"][[4, 50, 51, 299]] = 0
+    full = O.direct_gk(s, 1e-3)
+    idx = np.array([299, 0, 7, 50, 7, 123, 4, 298, 1], dtype=np.int32)
+    got = O.direct_targets(s, 1e-3, idx, nthreads=3)
+    for g, f in zip(got, full):
+        np.testing.assert_array_equal(g, f[idx])
+    ref = brute_acc_ld(s, 1e-3)[:, idx]
+    act = s["active"][idx].astype(bool)
+    assert rel(np.array(got)[:, act], ref[:, act]).max() < 1e-13
+    assert all((np.asarray(g)[~act] == 0).all() for g in got)
+    # the Plummer case the benchmarks judge with (softening 1e-4, 2000 bodies)
+    import cmhip
+    from conftest import soa_to_state
+    p = soa_to_state(cmhip.generate_plummer(2000, 1.0, 1.0, seed=5, mass_fraction_cut=0.999))
+    t = np.random.default_rng(4).choice(2000, 64, replace=False).astype(np.int32)
+    for g, f in zip(O.direct_targets(p, 1e-4, t), O.direct_gk(p, 1e-4)):
+        np.testing.assert_array_equal(g, f[t])
+
+
 def test_newton_third_law_momentum():
     s = random_state(200, seed=4)
     ax, ay, az = O.direct_n3(s, 1e-3)
@@ -458,3 +485,19 @@ def test_groups_cover_bodies_in_oversized_leaves(policy):
     ref = O.direct_gk(s, 1e-4)
     assert rel(a, ref).max() < 1e-2
     assert (st["n_parts"] + st["n_cells"] > 0).all()
+
+
+def test_pair_collapse_state_overflows_only_after_the_first_drift():
+    """The state of the sticky pool-overflow GPU test: with leaves of one body
+    and 21-bit keys, the tree after the first drift needs more nodes than the
+    library's pool (3 n + 1024), the tree after the second drift fewer."""
+    from conftest import pair_collapse_state, soa_to_state
+    soa = pair_collapse_state()
+    s = soa_to_state(soa)
+    n = soa.count
+    pool = 3 * n + 1024
+    counts = []
+    for _ in range(2):
+        O.verlet_step(s, 1e-3, lambda st: (np.zeros(n), np.zeros(n), np.zeros(n)))
+        counts.append(O.tree_build(s, 21, 1, node_cap=40 * n)["n_nodes"])
+    assert counts[0] > pool > counts[1], (counts, pool)
