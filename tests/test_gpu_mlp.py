@@ -200,8 +200,8 @@ def test_ablation_at_other_thetas(theta):
     the median error with it is reported beside the plain traversal's.  At
     theta 0.5 the monopole residual of accepted cells is small and the
     correction is bounded by the group-level expansion; at 0.9 it is large.
-    The check is that the trained correction never makes forces worse by more
-    than 10 % (the gain is measured, not assumed)."""
+    Measured: 4.06e-4 -> 2.79e-4 at 0.5 (1.45x), 2.26e-3 -> 1.85e-3 at 0.9
+    (1.22x); the check asks for a 1.1x gain at each."""
     soa = cmhip.generate_merger(65536, seed=11)
     be = _backend(soa.count, theta)
     be.upload(soa)
@@ -217,4 +217,4 @@ def test_ablation_at_other_thetas(theta):
     learned = _median_error(be, soa, targets, exact)
     print(f"theta {theta}: median rel force error plain {plain:.3e}, with MLP {learned:.3e} "
           f"(x{plain / learned:.2f})", flush=True)
-    assert learned <= 1.1 * plain
+    assert learned < plain / 1.1
