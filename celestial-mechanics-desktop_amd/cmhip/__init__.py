@@ -63,7 +63,7 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_default_pn_config", "cm_set_post_newtonian", "cm_pn_warnings",
            "cm_default_softening_config", "cm_set_softening", "cm_yoshida4_step",
            "cm_default_collision_config", "cm_configure_collisions", "cm_resolve_collisions",
-           "cm_nbody_step")
+           "cm_nbody_step", "cm_energy_noalloc")
 
 COLLISION_MERGE_ONLY, COLLISION_BOUNCE_ONLY, COLLISION_REALISTIC = 0, 1, 2
 
@@ -181,6 +181,7 @@ def load_library():
         "cm_compute_forces": ([vp, f64], i32),
         "cm_verlet_step": ([vp, f64, f64, i32], i32),
         "cm_energy": ([vp, f64, i32, C.POINTER(EnergyReport)], i32),
+        "cm_energy_noalloc": ([vp, f64, i32, C.POINTER(EnergyReport)], i32),
         "cm_potential_rows": ([vp, f64, i32, C.POINTER(C.c_double)], i32),
         "cm_max_acceleration": ([vp, C.POINTER(C.c_double)], i32),
         "cm_direct_sample": ([vp, f64, vp, i32, vp, vp, vp], i32),
@@ -502,11 +503,13 @@ class HipPhysicsBackend:
     def reset_stats(self):
         self._check(self._L.cm_reset_force_stats(self._h), "cm_reset_force_stats")
 
-    def energy(self, softening, with_potential=True):
-        """with_potential: False/0 none, True/1 exact pair sum, 2 Barnes-Hut."""
+    def energy(self, softening, with_potential=True, noalloc=False):
+        """with_potential: False/0 none, True/1 exact pair sum, 2 Barnes-Hut.
+        noalloc: use cm_energy_noalloc (no device allocation; same results)."""
         r = EnergyReport()
-        self._check(self._L.cm_energy(self._h, float(softening), int(with_potential), C.byref(r)),
-                    "cm_energy")
+        fn = self._L.cm_energy_noalloc if noalloc else self._L.cm_energy
+        self._check(fn(self._h, float(softening), int(with_potential), C.byref(r)),
+                    "cm_energy_noalloc" if noalloc else "cm_energy")
         return {"kinetic": r.kinetic, "potential": r.potential,
                 "momentum": tuple(r.momentum), "angular_momentum": tuple(r.angular_momentum),
                 "angular_momentum_mag": r.angular_momentum_mag,

@@ -19,6 +19,10 @@ constexpr int kStackCap = 1536;         // per-wave LDS traversal stack (ints)
 constexpr int kSortTile = 2048;         // keys per radix-sort tile
 constexpr int kSortThreads = 256;
 constexpr int kTreeTile = 1024;         // bodies per tree-build tile
+constexpr int kMaxDevices = 64;         // per-device tables of static device memory
+// largest body count of cm_energy_noalloc with a potential (its per-body rows
+// live in a static device array of this many doubles, 32 MiB per GPU)
+constexpr int kNoAllocMaxBodies = 1 << 22;
 // FP32 pair loops may leave a body's own (dx = 0) term in the sum when the
 // softening makes it exactly zero: r^2 = eps^2, rsq(eps^2)^3 <= 1e18 for
 // eps^2 >= 1e-12, so m rsq^3 stays finite for any mass below 3e20 and the
@@ -114,6 +118,8 @@ struct Engine {
 
     // diagnostics scratch
     double* red = nullptr;             // reduction partials
+    double* noalloc_scratch = nullptr; // device address of the static scratch of
+                                       // cm_energy_noalloc on this engine's GPU
     double* sample = nullptr;          // direct-sample partials
 
     // timing
@@ -147,6 +153,8 @@ int launch_barnes_hut(Engine& e, double softening);
 int launch_direct_sample(Engine& e, double softening, const int32_t* targets,
                          int32_t nt, double* ax, double* ay, double* az);
 int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out);
+int launch_energy_noalloc(Engine& e, double softening, int with_pe, cm_energy_report* out);
+int noalloc_scratch_init(Engine& e);
 int launch_bh_potential(Engine& e, double softening, double* rows);
 int potential_rows(Engine& e, double softening, int method, double* rows);
 int max_acceleration(Engine& e, double* out);

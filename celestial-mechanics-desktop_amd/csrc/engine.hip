@@ -332,6 +332,8 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     if (hipMemsetAsync(e->hdr, 0, sizeof(cm::TreeHeader), e->stream) != hipSuccess ||
         hipStreamSynchronize(e->stream) != hipSuccess)
         return fail(CM_ERR_HIP);
+    // device address of cm_energy_noalloc's static scratch on this GPU
+    if (int rc = cm::noalloc_scratch_init(*e); rc != CM_OK) return fail(rc);
     *out_engine = e;
     return CM_OK;
 }
@@ -544,6 +546,17 @@ int32_t cm_energy(void* h, double softening, int32_t with_potential,
         return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "with_potential must be 0, 1 or 2");
     CM_HIP_TRY(*e, hipSetDevice(e->device));
     return cm::launch_energy(*e, softening, with_potential, out);
+}
+
+int32_t cm_energy_noalloc(void* h, double softening, int32_t with_potential,
+                          cm_energy_report* out) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!out) return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "null output");
+    if (with_potential < 0 || with_potential > 2)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "with_potential must be 0, 1 or 2");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    return cm::launch_energy_noalloc(*e, softening, with_potential, out);
 }
 
 int32_t cm_potential_rows(void* h, double softening, int32_t method, double* rows) {

@@ -21,6 +21,9 @@
 // which it is taken, so that result is exact.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "cm_internal.h"
 
 namespace cm {
@@ -187,30 +190,31 @@ int potential_rows(Engine& e, double softening, int method, double* host_rows) {
     return rc;
 }
 
-int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out) {
+// KE, P, L, mass, COM (and PE from per-body rows when with_pe > 0) with the
+// per-body rows in `rows` (count doubles) and the reduction partials in `red`
+// (kEnergyRedDoubles doubles); both device buffers owned by the caller.
+constexpr size_t kEnergyRedDoubles = 4 * (size_t)kQ * kRedBlocks + 2 * kQ + 64;
+static_assert(kEnergyRedDoubles <= 64 * 1024, "energy partials fit the engine's diagnostics scratch");
+
+static int energy_into(Engine& e, double softening, int with_pe, double* rows, double* red,
+                       cm_energy_report* out) {
     const int n = e.count;
-    double* rows = nullptr;
     hipStream_t st = e.stream;
     if (with_pe && n > 0) {
-        CM_HIP_TRY(e, hipMalloc(&rows, (size_t)n * 8));
         int rc = launch_rows(e, softening, with_pe, rows);
         if (rc == CM_OK && with_pe == 2) rc = check_pool(e);
-        if (rc != CM_OK) {
-            (void)hipFree(rows);
-            return rc;
-        }
+        if (rc != CM_OK) return rc;
     }
-    Comp* part = reinterpret_cast<Comp*>(e.red);
-    double* res = e.red + 2 * kQ * kRedBlocks;
+    Comp* part = reinterpret_cast<Comp*>(red);
+    double* res = red + 2 * kQ * kRedBlocks;
     hipLaunchKernelGGL(k_energy_bodies, dim3(kRedBlocks), dim3(256), 0, st, n,
                        e.f[PX], e.f[PY], e.f[PZ], e.f[VX], e.f[VY], e.f[VZ],
-                       e.f[MASS], e.act, rows, part);
+                       e.f[MASS], e.act, with_pe ? rows : nullptr, part);
     hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, kRedBlocks, part, res);
     hipError_t err = hipGetLastError();
     double h[kQ] = {0};
     if (err == hipSuccess) err = hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st);
     if (err == hipSuccess) err = hipStreamSynchronize(st);
-    if (rows) (void)hipFree(rows);
     if (err != hipSuccess) return set_error(e, CM_ERR_HIP, hipGetErrorString(err));
     out->kinetic = h[0];
     out->momentum[0] = h[1]; out->momentum[1] = h[2]; out->momentum[2] = h[3];
@@ -223,6 +227,56 @@ int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* ou
     out->active_count = (int32_t)h[9];
     for (int k = 0; k < 3; k++) out->center_of_mass[k] = h[7] > 0.0 ? h[10 + k] / h[7] : 0.0;
     return CM_OK;
+}
+
+int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out) {
+    const int n = e.count;
+    double* rows = nullptr;
+    if (with_pe && n > 0) CM_HIP_TRY(e, hipMalloc(&rows, (size_t)n * 8));
+    int rc = energy_into(e, softening, with_pe, rows, e.red, out);
+    if (rows) (void)hipFree(rows);
+    return rc;
+}
+
+// ---- the allocation-free variant (cm_energy_noalloc) ------------------------
+// Its per-body potential rows and reduction partials live in this file-scope
+// device array (one copy per GPU, held until the process ends).  Host code
+// never passes the array's name to a kernel: it reads the array's device
+// address once per device with hipGetSymbolAddress (noalloc_scratch_init, at
+// cm_create) and uses that address.  Every call on one GPU shares the array.
+constexpr size_t kNoAllocRowDoubles = (size_t)kNoAllocMaxBodies;
+static __device__ __align__(256) double scratch[kNoAllocRowDoubles + kEnergyRedDoubles];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch_init(Engine& e) {
+    if (e.device < 0 || e.device >= kMaxDevices)
+        return set_error(e, CM_ERR_DEVICE_NOT_FOUND, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[e.device]) {
+        void* p = nullptr;
+        hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p)
+            return set_error(e, CM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                                 hipGetErrorString(err));
+        g_scratch_ptr[e.device] = static_cast<double*>(p);
+    }
+    e.noalloc_scratch = g_scratch_ptr[e.device];
+    return CM_OK;
+}
+
+int launch_energy_noalloc(Engine& e, double softening, int with_pe, cm_energy_report* out) {
+    if (!e.noalloc_scratch)
+        return set_error(e, CM_ERR_NOT_INITIALIZED, "scratch address not initialised");
+    if (with_pe && e.count > kNoAllocMaxBodies)
+        return set_error(e, CM_ERR_INVALID_ARGUMENT,
+                         "cm_energy_noalloc with a potential holds at most " +
+                             std::to_string(kNoAllocMaxBodies) + " bodies (count " +
+                             std::to_string(e.count) + ")");
+    double* rows = e.noalloc_scratch;
+    double* red = e.noalloc_scratch + kNoAllocRowDoubles;
+    return energy_into(e, softening, with_pe, rows, red, out);
 }
 
 __global__ void __launch_bounds__(256)

@@ -185,6 +185,18 @@ int32_t cm_verlet_step(void* engine, double softening, double dt,
  * theorem), on a tree built for the current positions. */
 int32_t cm_energy(void* engine, double softening, int32_t with_potential,
                   cm_energy_report* out);
+/* cm_energy that allocates nothing (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * outputs and results as cm_energy (bit for bit), but the per-body potential
+ * rows and the reduction partials live in a static device array of the
+ * library (32 MiB + 108 KiB per GPU, kept until the process ends) instead of
+ * a per-call hipMalloc.  with_potential > 0 holds at most 4,194,304 bodies
+ * (CM_ERR_INVALID_ARGUMENT beyond).  Every engine on one GPU shares that
+ * array: calls that overlap on one GPU (from several threads or engines)
+ * race and must be serialised by the caller. */
+int32_t cm_energy_noalloc(void* engine, double softening, int32_t with_potential,
+                          cm_energy_report* out);
 /* Per-body shares of the potential energy, rows[i] = 1/2 m_i phi_i (0 for
  * inactive bodies; host array of count doubles): method 1 exact, method 2
  * Barnes-Hut as in cm_energy.  The per-body m_i phi_i the engine's
