@@ -80,7 +80,8 @@ def run_path(cmhip, soa, precision, theta, steps, warmup, eps, dt, sample, param
     ax, ay, az = out.acc_x[sample], out.acc_y[sample], out.acc_z[sample]
     num = np.sqrt((ax - ex[0]) ** 2 + (ay - ex[1]) ** 2 + (az - ex[2]) ** 2)
     den = np.sqrt(ex[0] ** 2 + ex[1] ** 2 + ex[2] ** 2)
-    err = num / den
+    keep = den >= 1e-30     # ForceErrorAnalyzer.cs:108 skips near-zero exact forces
+    err = num[keep] / den[keep]
     be.close()
     return {
         "body_steps_per_s": n * steps / wall,
@@ -143,9 +144,34 @@ def train_network(cmhip, soa, theta, eps, n_sub=65536, seed=2024, epochs=20):
                     "target_bodies_approx": int(round(max_groups * n / n_groups))}
 
 
-def cpu_baseline(soa, eps, dt, threads):
-    """Reference CPU path restated in C (oracle): BarnesHutBackend with serial
-    insertion + OpenMP traversal inside SoAVerletIntegrator.Step, one step."""
+def host_cpu():
+    """Host cores available to this process and the CPU model."""
+    nproc = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = nproc
+    threads = avail
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        threads = min(threads, int(env))
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity": avail, "threads": threads, "model": model}
+
+
+def cpu_baseline(soa, eps, dt, host, steps=3, warmup=1):
+    """Reference CPU path restated in C (oracle): SoAVerletIntegrator.Step around
+    BarnesHutBackend.ComputeForces (serial insertion, OpenMP traversal on every
+    available host core standing in for Parallel.For), `warmup` untimed steps
+    then `steps` timed steps on the same N = 2^20 state."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     n = soa.count
@@ -155,13 +181,77 @@ def cpu_baseline(soa, eps, dt, threads):
          "oax": np.zeros(n), "oay": np.zeros(n), "oaz": np.zeros(n),
          "mass": soa.mass[:n].copy(), "radius": np.zeros(n),
          "active": np.ones(n, dtype=np.uint8)}
+    threads = host["threads"]
+
+    def force(st):
+        return O.bh_csharp(st, eps, 0.5, nthreads=threads)[0]
+
+    for _ in range(warmup):
+        O.verlet_step(s, dt, force)
     t0 = time.perf_counter()
-    O.verlet_step(s, dt, lambda st: O.bh_csharp(st, eps, 0.5, nthreads=threads)[0])
-    t = time.perf_counter() - t0
+    for _ in range(steps):
+        O.verlet_step(s, dt, force)
+    t = (time.perf_counter() - t0) / steps
     return {"value": n / t, "unit": "body-steps/s", "cores": threads, "kind": "port",
-            "sample": f"1 Verlet step of the same N={n} Plummer state, C# BarnesHutBackend "
-                      f"restated in C (serial insertion, OpenMP traversal, theta=0.5)",
-            "seconds": t}
+            "sample": f"{steps} timed Verlet steps after {warmup} warm-up of the same N={n} "
+                      f"Plummer state, C# BarnesHutBackend restated in C (serial insertion, "
+                      f"OpenMP traversal on {threads} threads, theta=0.5)",
+            "seconds_per_step": t, "nproc": host["nproc"], "affinity_cores": host["affinity"],
+            "cpu_model": host["model"]}
+
+
+def config1_direct(cmhip, steps=1000, cpu_steps=100):
+    """The plain method (CPU-runnable configuration): N = 1000 Plummer
+    (M = 1, a = 1, SplitMix64 seed 42, centre-of-mass frame), FP64 direct sum
+    in CpuSingleThreadBackend's Newton-3 order (k_direct_pairwise), softening
+    1e-4, Verlet dt = 1e-3, `steps` device-resident steps: ms/step, energy
+    drift, and the force error of the first evaluation against the FP64 full
+    sweep (CpuParallelBackend order) and the oracle's Newton-3 restatement
+    (bit for bit).  Beside it the oracle's single-thread restatement of
+    CpuSingleThreadBackend + SoAVerletIntegrator, timed over `cpu_steps`."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    n, eps, dt = 1000, 1e-4, 1e-3
+    soa = cmhip.generate_plummer(n, seed=42)
+    st = {"px": soa.pos_x[:n].copy(), "py": soa.pos_y[:n].copy(), "pz": soa.pos_z[:n].copy(),
+          "mass": soa.mass[:n].copy(), "radius": np.zeros(n), "active": np.ones(n, np.uint8)}
+    be = cmhip.HipPhysicsBackend(n, kind=cmhip.BACKEND_DIRECT_PAIRWISE, precision=cmhip.PREC_FP64)
+    work = soa.copy()
+    be.compute_forces(work, eps)
+    got = (work.acc_x[:n], work.acc_y[:n], work.acc_z[:n])
+    sweep = O.direct_gk(st, eps)
+    err = O.force_error(sweep, got)
+    bit_exact = all(np.array_equal(a, b) for a, b in zip(got, O.direct_n3(st, eps)))
+    be.upload(soa)
+    e = be.energy(eps, 1)
+    e0 = e["kinetic"] + e["potential"]
+    be.step(eps, dt, 10)                  # warm-up (kernels loaded)
+    be.upload(soa)
+    t0 = time.perf_counter()
+    be.step(eps, dt, steps)               # synchronises at the end
+    wall = time.perf_counter() - t0
+    e = be.energy(eps, 1)
+    drift = (e["kinetic"] + e["potential"] - e0) / abs(e0)
+    be.close()
+    s = {"px": st["px"].copy(), "py": st["py"].copy(), "pz": st["pz"].copy(),
+         "vx": soa.vel_x[:n].copy(), "vy": soa.vel_y[:n].copy(), "vz": soa.vel_z[:n].copy(),
+         "ax": np.zeros(n), "ay": np.zeros(n), "az": np.zeros(n),
+         "oax": np.zeros(n), "oay": np.zeros(n), "oaz": np.zeros(n),
+         "mass": st["mass"].copy(), "active": np.ones(n, np.uint8)}
+    t0 = time.perf_counter()
+    for _ in range(cpu_steps):
+        O.verlet_step(s, dt, lambda x: O.direct_n3(x, eps))
+    cpu = (time.perf_counter() - t0) / cpu_steps
+    pairs = n * (n - 1) // 2
+    return {"workload": "plummer1000_direct_fp64_verlet", "n_bodies": n, "steps": steps,
+            "ms_per_step": 1e3 * wall / steps, "body_steps_per_s": n * steps / wall,
+            "energy_drift": drift, "force_err_vs_fp64_sweep": {k: err[k] for k in ("median", "max")},
+            "bit_exact_vs_cpu_single_thread": bit_exact,
+            "cpu_baseline": {"value": n / cpu, "unit": "body-steps/s", "cores": 1, "kind": "port",
+                             "ms_per_step": 1e3 * cpu,
+                             "sample": f"{cpu_steps} steps of CpuSingleThreadBackend "
+                                       f"(Newton-3, {pairs} pairs) + SoAVerletIntegrator "
+                                       f"restated in C, one thread"}}
 
 
 def traffic_from_profile():
@@ -629,10 +719,14 @@ def main():
         if "without_network" in rec:
             line[key]["without_network"] = rec["without_network"]
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline on {threads} threads ...")
-        cb = cpu_baseline(soa, eps, dt, threads)
-        line["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        cpu_host = host = host_cpu()
+        log(f"cpu baseline on {host['threads']} threads ({host['model']}) ...")
+        cb = cpu_baseline(soa, eps, dt, cpu_host)
+        line["cpu_baseline"] = cb
+    if not args.fast_only:
+        log("config 1: N=1000 direct FP64, 1000 steps ...")
+        line["config1_direct"] = config1_direct(cmhip)
+        log(json.dumps(line["config1_direct"]))
     print(json.dumps(line), flush=True)
 
 

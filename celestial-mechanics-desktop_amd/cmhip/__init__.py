@@ -63,7 +63,8 @@ EXPORTS = ("cm_create", "cm_destroy", "cm_last_error", "cm_configure",
            "cm_default_pn_config", "cm_set_post_newtonian", "cm_pn_warnings",
            "cm_default_softening_config", "cm_set_softening", "cm_yoshida4_step",
            "cm_default_collision_config", "cm_configure_collisions", "cm_resolve_collisions",
-           "cm_nbody_step", "cm_energy_noalloc")
+           "cm_nbody_step", "cm_energy_noalloc", "cm_compare_forces", "cm_nbody_step_diag",
+           "cm_upload_force_inputs")
 
 COLLISION_MERGE_ONLY, COLLISION_BOUNCE_ONLY, COLLISION_REALISTIC = 0, 1, 2
 
@@ -135,6 +136,32 @@ class ForceStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ForceComparison(C.Structure):
+    """ForceErrorAnalyzer.ComparisonResult (Validation/ForceErrorAnalyzer.cs:67-135)."""
+    _fields_ = [("mean_relative_error", C.c_double), ("max_relative_error", C.c_double),
+                ("rms_relative_error", C.c_double), ("median_relative_error", C.c_double),
+                ("body_count", C.c_int32), ("theta", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SimulationState(C.Structure):
+    """Types/SimulationState.cs:9-25, as NBodySolver.Step returns it."""
+    _fields_ = [("time", C.c_double), ("body_count", C.c_int32),
+                ("active_body_count", C.c_int32), ("kinetic_energy", C.c_double),
+                ("potential_energy", C.c_double), ("total_energy", C.c_double),
+                ("total_momentum", C.c_double * 3), ("energy_drift", C.c_double),
+                ("collision_count", C.c_int32), ("current_dt", C.c_double),
+                ("step_energy_delta", C.c_double), ("step_momentum_delta", C.c_double),
+                ("step_angular_momentum_delta", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["total_momentum"] = tuple(self.total_momentum)
+        return d
+
+
 class EnergyReport(C.Structure):
     _fields_ = [("kinetic", C.c_double), ("potential", C.c_double),
                 ("momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
@@ -177,6 +204,7 @@ def load_library():
         "cm_default_config": ([C.POINTER(BackendConfig)], i32),
         "cm_upload_soa": ([vp, C.POINTER(SoAPtrs), i32], i32),
         "cm_download_soa": ([vp, C.POINTER(SoAPtrs), i32], i32),
+        "cm_upload_force_inputs": ([vp, C.POINTER(SoAPtrs), i32], i32),
         "cm_get_count": ([vp, C.POINTER(i32)], i32),
         "cm_compute_forces": ([vp, f64], i32),
         "cm_verlet_step": ([vp, f64, f64, i32], i32),
@@ -186,6 +214,7 @@ def load_library():
         "cm_max_acceleration": ([vp, C.POINTER(C.c_double)], i32),
         "cm_direct_sample": ([vp, f64, vp, i32, vp, vp, vp], i32),
         "cm_get_force_stats": ([vp, C.POINTER(ForceStats)], i32),
+        "cm_compare_forces": ([vp, f64, f64, vp, i32, C.POINTER(ForceComparison)], i32),
         "cm_reset_force_stats": ([vp], i32),
         "cm_detect_collisions": ([vp, vp, i32, C.POINTER(i32)], i32),
         "cm_resolve_merges": ([vp, C.POINTER(i32)], i32),
@@ -224,6 +253,7 @@ def load_library():
         "cm_configure_collisions": ([vp, C.POINTER(CollisionConfig)], i32),
         "cm_resolve_collisions": ([vp, C.POINTER(i32)], i32),
         "cm_nbody_step": ([vp, f64, f64, i32, i32, C.POINTER(i32)], i32),
+        "cm_nbody_step_diag": ([vp, f64, f64, i32, i32, i32, vp], i32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -419,9 +449,14 @@ class HipPhysicsBackend:
 
     # -- IPhysicsComputeBackend ---------------------------------------------
     def compute_forces(self, bodies, softening):
-        self.upload(bodies)
+        self.upload_force_inputs(bodies)
         self._check(self._L.cm_compute_forces(self._h, float(softening)), "cm_compute_forces")
         self.download(bodies, fields=("acc_x", "acc_y", "acc_z"))
+
+    def upload_force_inputs(self, bodies):
+        """Only what ComputeForces reads (cm_upload_force_inputs)."""
+        self._check(self._L.cm_upload_force_inputs(self._h, C.byref(bodies.ptrs()), bodies.count),
+                    "cm_upload_force_inputs")
 
     # -- device-resident interface ----------------------------------------
     def upload(self, bodies):
@@ -458,6 +493,15 @@ class HipPhysicsBackend:
         self._check(self._L.cm_nbody_step(self._h, float(softening), float(dt), int(n_steps),
                                           int(bool(collisions)), C.byref(k)), "cm_nbody_step")
         return k.value
+
+    def nbody_step_diag(self, softening, dt, n_steps=1, collisions=False, with_potential=1):
+        """n_steps of NBodySolver.Step: returns the SimulationState of each step."""
+        st = (SimulationState * max(1, int(n_steps)))()
+        self._check(self._L.cm_nbody_step_diag(self._h, float(softening), float(dt),
+                                               int(n_steps), int(bool(collisions)),
+                                               int(with_potential), C.cast(st, C.c_void_p)),
+                    "cm_nbody_step_diag")
+        return [st[k].as_dict() for k in range(int(n_steps))]
 
     def yoshida_step(self, softening, dt, n_steps=1):
         """n_steps of the engine's Yoshida4 integrator (three Verlet sub-steps)."""
@@ -540,6 +584,18 @@ class HipPhysicsBackend:
                                              ax.ctypes.data, ay.ctypes.data, az.ctypes.data),
                     "cm_direct_sample")
         return ax, ay, az
+
+    def compare_forces(self, softening, theta, targets=None):
+        """ForceErrorAnalyzer.CompareForces on the device state: exact FP64
+        direct sum vs the Barnes-Hut backend at theta (all bodies, or the
+        given target indices); leaves the Barnes-Hut forces in acc."""
+        r = ForceComparison()
+        t = None if targets is None else np.ascontiguousarray(targets, dtype=np.int32)
+        self._check(self._L.cm_compare_forces(self._h, float(softening), float(theta),
+                                              None if t is None else t.ctypes.data,
+                                              0 if t is None else len(t), C.byref(r)),
+                    "cm_compare_forces")
+        return r.as_dict()
 
     def detect_collisions(self, cap=1 << 16):
         out = np.zeros(cap, dtype=EVENT_DTYPE)

@@ -118,6 +118,11 @@ struct Engine {
 
     // diagnostics scratch
     double* red = nullptr;             // reduction partials
+    // NBodySolver's diagnostics state (_currentTime, _initialEnergy), reset
+    // when new bodies are uploaded
+    double sim_time = 0.0;
+    double initial_energy = 0.0;
+    bool initial_energy_set = false;
     double* noalloc_scratch = nullptr; // device address of the static scratch of
                                        // cm_energy_noalloc on this engine's GPU
     double* sample = nullptr;          // direct-sample partials
@@ -155,6 +160,11 @@ int launch_direct_sample(Engine& e, double softening, const int32_t* targets,
 int launch_energy(Engine& e, double softening, int with_pe, cm_energy_report* out);
 int launch_energy_noalloc(Engine& e, double softening, int with_pe, cm_energy_report* out);
 int noalloc_scratch_init(Engine& e);
+int nbody_step_diag(Engine& e, double softening, double dt, int n_steps, int collisions,
+                    int with_pe, int (*verlet_one)(Engine&, double, double),
+                    cm_simulation_state* states);
+int compare_forces(Engine& e, double softening, double theta, const int32_t* targets,
+                   int32_t nt, cm_force_comparison* out);
 int launch_bh_potential(Engine& e, double softening, double* rows);
 int potential_rows(Engine& e, double softening, int method, double* rows);
 int max_acceleration(Engine& e, double* out);

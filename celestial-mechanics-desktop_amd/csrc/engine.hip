@@ -210,6 +210,10 @@ static int verlet_run(Engine& e, double softening, const double* dts, int n) {
     return rc;
 }
 
+static int verlet_one(Engine& e, double softening, double dt) {
+    return verlet_run(e, softening, &dt, 1);
+}
+
 static double* const* field_ptrs(const cm_soa_ptrs* p, double* out[NF64]) {
     double* const src[NF64] = {p->pos_x, p->pos_y, p->pos_z, p->vel_x,
                                p->vel_y, p->vel_z, p->acc_x, p->acc_y,
@@ -415,6 +419,49 @@ int32_t cm_upload_soa(void* h, const cm_soa_ptrs* src, int32_t count) {
         CM_HIP_TRY(*e, hipMemsetAsync(e->type, 0, (size_t)count * 4, e->stream));
     CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
     e->count = count;
+    e->sim_time = 0.0;
+    e->initial_energy = 0.0;
+    e->initial_energy_set = false;
+    return CM_OK;
+}
+
+int32_t cm_upload_force_inputs(void* h, const cm_soa_ptrs* src, int32_t count) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!src || count < 0)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "null source or negative count");
+    if (count > e->capacity)
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "body count exceeds capacity");
+    const bool need_vel = e->pn.enabled != 0;
+    const bool need_type = e->soft.mode == CM_SOFTENING_PER_TYPE;
+    const bool need_rad = e->cfg.shell_theorem != 0;
+    if (!src->pos_x || !src->pos_y || !src->pos_z || !src->mass || !src->is_active ||
+        (need_rad && !src->radius) ||
+        (need_vel && (!src->vel_x || !src->vel_y || !src->vel_z)) ||
+        (need_type && !src->body_type))
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT,
+                             "missing an array the force evaluation reads");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    const size_t b8 = (size_t)count * 8;
+    const double* from[cm::NF64] = {src->pos_x, src->pos_y, src->pos_z, src->vel_x,
+                                    src->vel_y, src->vel_z, nullptr, nullptr, nullptr,
+                                    nullptr, nullptr, nullptr, src->mass, src->radius,
+                                    nullptr};
+    for (int k = 0; k < cm::NF64; k++) {
+        if (!from[k]) continue;
+        if (k >= cm::VX && k <= cm::VZ && !need_vel) continue;
+        CM_HIP_TRY(*e, hipMemcpyAsync(e->f[k], from[k], b8, hipMemcpyHostToDevice, e->stream));
+    }
+    CM_HIP_TRY(*e, hipMemcpyAsync(e->act, src->is_active, count, hipMemcpyHostToDevice, e->stream));
+    if (need_type)
+        CM_HIP_TRY(*e, hipMemcpyAsync(e->type, src->body_type, (size_t)count * 4,
+                                      hipMemcpyHostToDevice, e->stream));
+    CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
+    if (count != e->count) {
+        e->sim_time = 0.0;
+        e->initial_energy_set = false;
+    }
+    e->count = count;
     return CM_OK;
 }
 
@@ -590,6 +637,21 @@ int32_t cm_direct_sample(void* h, double softening, const int32_t* targets,
     return cm::launch_direct_sample(*e, softening, targets, n_targets, ax, ay, az);
 }
 
+int32_t cm_compare_forces(void* h, double softening, double theta, const int32_t* targets,
+                          int32_t n_targets, cm_force_comparison* out) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!out || n_targets < 0 || (n_targets > 0 && !targets))
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "bad comparison arguments");
+    if (!(softening >= 0.0) || !(theta >= 0.0) || !(theta < 1e6))
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "bad softening or theta");
+    for (int t = 0; t < n_targets; t++)
+        if (targets[t] < 0 || targets[t] >= e->count)
+            return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT, "target index out of range");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    return cm::compare_forces(*e, softening, theta, targets, n_targets, out);
+}
+
 int32_t cm_get_force_stats(void* h, cm_force_stats* out) {
     Engine* e = as_engine(h);
     if (!e) return CM_ERR_NOT_INITIALIZED;
@@ -661,6 +723,27 @@ int32_t cm_nbody_step(void* h, double softening, double dt, int32_t n_steps,
     }
     if (n_resolved) *n_resolved = total;
     int rc = cm::finalize_stats(*e);
+    if (rc != CM_OK) return rc;
+    if (e->stats.flags & 1)
+        return cm::set_error(*e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");
+    return CM_OK;
+}
+
+int32_t cm_nbody_step_diag(void* h, double softening, double dt, int32_t n_steps,
+                           int32_t collisions, int32_t with_potential,
+                           cm_simulation_state* states) {
+    Engine* e = as_engine(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (!(softening >= 0.0) || n_steps < 0 || (collisions != 0 && collisions != 1) ||
+        with_potential < 0 || with_potential > 2 || (n_steps > 0 && !states))
+        return cm::set_error(*e, CM_ERR_INVALID_ARGUMENT,
+                             "bad softening, step count, flag, potential mode or output");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    if (int rc0 = cm::begin_run(*e); rc0 != CM_OK) return rc0;
+    int rc = cm::nbody_step_diag(*e, softening, dt, n_steps, collisions, with_potential,
+                                 cm::verlet_one, states);
+    if (rc != CM_OK) return rc;
+    rc = cm::finalize_stats(*e);
     if (rc != CM_OK) return rc;
     if (e->stats.flags & 1)
         return cm::set_error(*e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");

@@ -909,12 +909,15 @@ int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index) {
     hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, st, nb, c->scan);
     hipLaunchKernelGGL(k_new_index, dim3(nb), dim3(256), 0, st, n, e.act, c->scan,
                        c->new_index);
-    // every array goes through the sort buffer ka (n doubles / n x 8 bytes)
-    double* tmp = reinterpret_cast<double*>(c->ka);
+    // each double array is scattered into the sort buffer ka (capacity x 8
+    // bytes, the size of a body array), which then takes the array's place:
+    // the old array becomes the next scratch, so every array moves once
     for (int k = 0; k < NF64; k++) {
+        double* tmp = reinterpret_cast<double*>(c->ka);
         hipLaunchKernelGGL(k_scatter<double>, dim3(nb), dim3(256), 0, st, n, c->new_index,
                            e.f[k], tmp);
-        CM_HIP_TRY(e, hipMemcpyAsync(e.f[k], tmp, (size_t)n * 8, hipMemcpyDeviceToDevice, st));
+        c->ka = reinterpret_cast<uint64_t*>(e.f[k]);
+        e.f[k] = tmp;
     }
     uint8_t* t8 = reinterpret_cast<uint8_t*>(c->ka);
     hipLaunchKernelGGL(k_scatter<uint8_t>, dim3(nb), dim3(256), 0, st, n, c->new_index, e.act, t8);

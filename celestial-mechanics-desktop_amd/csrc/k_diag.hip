@@ -21,8 +21,10 @@
 // which it is taken, so that result is exact.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "cm_internal.h"
 
@@ -99,17 +101,18 @@ k_energy_bodies(int n, const double* __restrict__ px, const double* __restrict__
 }
 
 __global__ void __launch_bounds__(256)
-k_energy_final(int nblocks, const Comp* __restrict__ part, double* __restrict__ res) {
+k_energy_final(int nblocks, const Comp* __restrict__ part, Comp* __restrict__ tmp,
+               double* __restrict__ res) {
     Comp v[kQ];
     for (int q = 0; q < kQ; q++) {
         v[q] = Comp{0.0, 0.0};
         for (int b = threadIdx.x; b < nblocks; b += 256) v[q] = comp_merge(v[q], part[(size_t)q * kRedBlocks + b]);
     }
-    block_reduce_store(v, reinterpret_cast<Comp*>(res + 2 * kQ));
+    block_reduce_store(v, tmp);
     __syncthreads();
     if (threadIdx.x == 0)
         for (int q = 0; q < kQ; q++) {
-            Comp c = reinterpret_cast<Comp*>(res + 2 * kQ)[(size_t)q * kRedBlocks];
+            Comp c = tmp[(size_t)q * kRedBlocks];
             res[q] = c.s + c.c;
         }
 }
@@ -196,8 +199,9 @@ int potential_rows(Engine& e, double softening, int method, double* host_rows) {
 constexpr size_t kEnergyRedDoubles = 4 * (size_t)kQ * kRedBlocks + 2 * kQ + 64;
 static_assert(kEnergyRedDoubles <= 64 * 1024, "energy partials fit the engine's diagnostics scratch");
 
-static int energy_into(Engine& e, double softening, int with_pe, double* rows, double* red,
-                       cm_energy_report* out) {
+// the kQ sums of one evaluation into dev_out (device, kQ doubles), no host sync
+static int energy_sums_async(Engine& e, double softening, int with_pe, double* rows, double* red,
+                             double* dev_out) {
     const int n = e.count;
     hipStream_t st = e.stream;
     if (with_pe && n > 0) {
@@ -206,16 +210,16 @@ static int energy_into(Engine& e, double softening, int with_pe, double* rows, d
         if (rc != CM_OK) return rc;
     }
     Comp* part = reinterpret_cast<Comp*>(red);
-    double* res = red + 2 * kQ * kRedBlocks;
+    Comp* tmp = part + (size_t)kQ * kRedBlocks;
     hipLaunchKernelGGL(k_energy_bodies, dim3(kRedBlocks), dim3(256), 0, st, n,
                        e.f[PX], e.f[PY], e.f[PZ], e.f[VX], e.f[VY], e.f[VZ],
                        e.f[MASS], e.act, with_pe ? rows : nullptr, part);
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, kRedBlocks, part, res);
-    hipError_t err = hipGetLastError();
-    double h[kQ] = {0};
-    if (err == hipSuccess) err = hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, st);
-    if (err == hipSuccess) err = hipStreamSynchronize(st);
-    if (err != hipSuccess) return set_error(e, CM_ERR_HIP, hipGetErrorString(err));
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, st, kRedBlocks, part, tmp, dev_out);
+    CM_HIP_TRY(e, hipGetLastError());
+    return CM_OK;
+}
+
+static void fill_report(const double* h, int with_pe, cm_energy_report* out) {
     out->kinetic = h[0];
     out->momentum[0] = h[1]; out->momentum[1] = h[2]; out->momentum[2] = h[3];
     out->angular_momentum[0] = h[4];
@@ -226,6 +230,18 @@ static int energy_into(Engine& e, double softening, int with_pe, double* rows, d
     out->potential = with_pe ? h[8] : 0.0;
     out->active_count = (int32_t)h[9];
     for (int k = 0; k < 3; k++) out->center_of_mass[k] = h[7] > 0.0 ? h[10 + k] / h[7] : 0.0;
+}
+
+static int energy_into(Engine& e, double softening, int with_pe, double* rows, double* red,
+                       cm_energy_report* out) {
+    double* res = red + 4 * (size_t)kQ * kRedBlocks;
+    int rc = energy_sums_async(e, softening, with_pe, rows, red, res);
+    if (rc != CM_OK) return rc;
+    double h[kQ] = {0};
+    hipError_t err = hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, e.stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+    if (err != hipSuccess) return set_error(e, CM_ERR_HIP, hipGetErrorString(err));
+    fill_report(h, with_pe, out);
     return CM_OK;
 }
 
@@ -326,6 +342,211 @@ int max_acceleration(Engine& e, double* out) {
     CM_HIP_TRY(e, hipMemcpyAsync(out, e.red + kRedBlocks, 8, hipMemcpyDeviceToHost, e.stream));
     CM_HIP_TRY(e, hipStreamSynchronize(e.stream));
     return CM_OK;
+}
+
+// ---- NBodySolver.Step with its SimulationState ---------------------------------
+// n_steps x (StepSoA, then the diagnostics NBodySolver.Step computes around it,
+// Solvers/NBodySolver.cs:307-379).  The energy sums of the state before the
+// first step and after every step are reduced on the device into one history
+// buffer (no host sync between steps unless collisions are on); the states
+// are formed on the host at the end.
+int nbody_step_diag(Engine& e, double softening, double dt, int n_steps, int collisions,
+                    int with_pe, int (*verlet_one)(Engine&, double, double),
+                    cm_simulation_state* states) {
+    const int n = e.count;
+    double* hist = nullptr;
+    double* rows = nullptr;
+    if (hipMalloc(&hist, (size_t)(n_steps + 1) * kQ * 8) != hipSuccess)
+        return set_error(e, CM_ERR_OUT_OF_MEMORY, "diagnostics history");
+    if (with_pe && n > 0 && hipMalloc(&rows, (size_t)n * 8) != hipSuccess) {
+        (void)hipFree(hist);
+        return set_error(e, CM_ERR_OUT_OF_MEMORY, "potential rows");
+    }
+    std::vector<int32_t> resolved(n_steps, 0);
+    int rc = energy_sums_async(e, softening, with_pe, rows, e.red, hist);
+    for (int k = 0; k < n_steps && rc == CM_OK; k++) {
+        rc = verlet_one(e, softening, dt);
+        if (rc == CM_OK && collisions) {
+            int32_t ne = 0;
+            rc = collide_detect(e, nullptr, 0, &ne);
+            if (rc == CM_OK) rc = collide_resolve(e, e.coll_cfg, &resolved[k]);
+        }
+        if (rc == CM_OK)
+            rc = energy_sums_async(e, softening, with_pe, rows, e.red, hist + (size_t)(k + 1) * kQ);
+    }
+    std::vector<double> h((size_t)(n_steps + 1) * kQ);
+    if (rc == CM_OK) {
+        hipError_t err = hipMemcpyAsync(h.data(), hist, h.size() * 8, hipMemcpyDeviceToHost, e.stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+        if (err != hipSuccess) rc = set_error(e, CM_ERR_HIP, hipGetErrorString(err));
+    }
+    (void)hipFree(hist);
+    if (rows) (void)hipFree(rows);
+    if (rc != CM_OK) return rc;
+    for (int k = 0; k < n_steps; k++) {
+        cm_energy_report b{}, a{};
+        fill_report(&h[(size_t)k * kQ], with_pe, &b);
+        fill_report(&h[(size_t)(k + 1) * kQ], with_pe, &a);
+        const double eb = b.kinetic + b.potential, ea = a.kinetic + a.potential;
+        e.sim_time += dt;
+        if (!e.initial_energy_set) {   // NBodySolver.cs:345-349
+            e.initial_energy = ea;
+            e.initial_energy_set = true;
+        }
+        cm_simulation_state& s = states[k];
+        s = cm_simulation_state{};
+        s.time = e.sim_time;
+        s.body_count = n;
+        s.active_body_count = a.active_count;
+        s.kinetic_energy = a.kinetic;
+        s.potential_energy = a.potential;
+        s.total_energy = ea;
+        for (int q = 0; q < 3; q++) s.total_momentum[q] = a.momentum[q];
+        s.energy_drift = fabs(e.initial_energy) > 1e-15
+                             ? (ea - e.initial_energy) / fabs(e.initial_energy) : 0.0;
+        s.collision_count = resolved[k];
+        s.current_dt = dt;
+        s.step_energy_delta = eb - ea;
+        const double dpx = b.momentum[0] - a.momentum[0], dpy = b.momentum[1] - a.momentum[1],
+                     dpz = b.momentum[2] - a.momentum[2];
+        s.step_momentum_delta = sqrt(dpx * dpx + dpy * dpy + dpz * dpz);
+        s.step_angular_momentum_delta = fabs(b.angular_momentum_mag - a.angular_momentum_mag);
+    }
+    return CM_OK;
+}
+
+// ---- ForceErrorAnalyzer.CompareForces ----------------------------------------
+// Per-body relative error |a_test - a_exact| / |a_exact| (the reference's
+// arithmetic, ForceErrorAnalyzer.cs:98-116); -1 marks a body the analyzer
+// skips (inactive, or |a_exact| < 1e-30).
+__global__ void __launch_bounds__(256)
+k_rel_error(int n, const uint8_t* __restrict__ act, const double* __restrict__ ex,
+            const double* __restrict__ ey, const double* __restrict__ ez,
+            const double* __restrict__ tx, const double* __restrict__ ty,
+            const double* __restrict__ tz, double* __restrict__ rel) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    double r = -1.0;
+    if (act[i]) {
+        const double x = ex[i], y = ey[i], z = ez[i];
+        const double mag = sqrt(x * x + y * y + z * z);
+        if (!(mag < 1e-30)) {
+            const double dx = tx[i] - x, dy = ty[i] - y, dz = tz[i] - z;
+            r = sqrt(dx * dx + dy * dy + dz * dz) / mag;
+        }
+    }
+    rel[i] = r;
+}
+
+// mean / max / RMS accumulated in body order as the reference's loop does
+// (ForceErrorAnalyzer.cs:97-124), plus the median
+static void error_statistics(std::vector<double>& errs, cm_force_comparison* out) {
+    double sum = 0.0, sum2 = 0.0, mx = 0.0;
+    for (double r : errs) {
+        sum += r;
+        sum2 += r * r;
+        if (r > mx) mx = r;
+    }
+    const size_t c = errs.size();
+    out->body_count = (int32_t)c;
+    out->mean_relative_error = c ? sum / (double)c : 0.0;
+    out->rms_relative_error = c ? sqrt(sum2 / (double)c) : 0.0;
+    out->max_relative_error = mx;
+    out->median_relative_error = 0.0;
+    if (c) {
+        std::sort(errs.begin(), errs.end());
+        out->median_relative_error = (c & 1) ? errs[c / 2] : 0.5 * (errs[c / 2 - 1] + errs[c / 2]);
+    }
+}
+
+int compare_forces(Engine& e, double softening, double theta, const int32_t* targets,
+                   int32_t nt, cm_force_comparison* out) {
+    const int n = e.count;
+    *out = cm_force_comparison{};
+    out->theta = theta;
+    if (n == 0) return CM_OK;
+    const cm_backend_config saved = e.cfg;
+    const cm_softening_config saved_soft = e.soft;
+    const cm_pn_config saved_pn = e.pn;
+    // the analyzer's plain backends: global softening, no shell theorem, no 1PN
+    e.soft.mode = CM_SOFTENING_GLOBAL;
+    e.pn.enabled = 0;
+    e.cfg.shell_theorem = 0;
+    double* ex = nullptr;
+    double* rel = nullptr;
+    std::vector<double> hx, hy, hz;
+    auto restore = [&](int rc) {
+        e.cfg = saved;
+        e.soft = saved_soft;
+        e.pn = saved_pn;
+        if (ex) (void)hipFree(ex);
+        if (rel) (void)hipFree(rel);
+        return rc;
+    };
+    int rc = CM_OK;
+    if (nt == 0) {
+        // exact forces of every body: the FP64 direct sum (CpuParallelBackend
+        // order; CpuSingleThreadBackend's pair terms in a per-body order)
+        if (hipMalloc(&ex, (size_t)n * 3 * 8) != hipSuccess || hipMalloc(&rel, (size_t)n * 8) != hipSuccess)
+            return restore(set_error(e, CM_ERR_OUT_OF_MEMORY, "compare_forces buffers"));
+        e.cfg.kind = CM_BACKEND_DIRECT;
+        rc = launch_direct(e, softening);
+        for (int k = 0; k < 3 && rc == CM_OK; k++)
+            if (hipMemcpyAsync(ex + (size_t)k * n, e.f[AX + k], (size_t)n * 8,
+                               hipMemcpyDeviceToDevice, e.stream) != hipSuccess)
+                rc = set_error(e, CM_ERR_HIP, "copy of the exact forces");
+    } else {
+        hx.resize(nt); hy.resize(nt); hz.resize(nt);
+        rc = launch_direct_sample(e, softening, targets, nt, hx.data(), hy.data(), hz.data());
+    }
+    if (rc != CM_OK) return restore(rc);
+    // test forces: the configured Barnes-Hut backend at the analyzer's theta
+    e.cfg.kind = CM_BACKEND_BARNES_HUT;
+    e.cfg.theta = theta;
+    rc = begin_run(e);
+    if (rc == CM_OK) rc = launch_barnes_hut(e, softening);
+    if (rc == CM_OK) rc = finalize_stats(e);
+    if (rc == CM_OK && (e.stats.flags & 1))
+        rc = set_error(e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");
+    if (rc != CM_OK) return restore(rc);
+    std::vector<double> errs;
+    if (nt == 0) {
+        hipLaunchKernelGGL(k_rel_error, dim3((n + 255) / 256), dim3(256), 0, e.stream, n, e.act,
+                           ex, ex + n, ex + 2 * (size_t)n, e.f[AX], e.f[AY], e.f[AZ], rel);
+        std::vector<double> h(n);
+        hipError_t err = hipGetLastError();
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(h.data(), rel, (size_t)n * 8, hipMemcpyDeviceToHost, e.stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+        if (err != hipSuccess) return restore(set_error(e, CM_ERR_HIP, hipGetErrorString(err)));
+        errs.reserve(n);
+        for (int i = 0; i < n; i++)
+            if (h[i] >= 0.0) errs.push_back(h[i]);
+    } else {
+        std::vector<double> ax(n), ay(n), az(n);
+        std::vector<uint8_t> act(n);
+        hipError_t err = hipSuccess;
+        double* dst[3] = {ax.data(), ay.data(), az.data()};
+        for (int k = 0; k < 3 && err == hipSuccess; k++)
+            err = hipMemcpyAsync(dst[k], e.f[AX + k], (size_t)n * 8, hipMemcpyDeviceToHost, e.stream);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(act.data(), e.act, (size_t)n, hipMemcpyDeviceToHost, e.stream);
+        if (err == hipSuccess) err = hipStreamSynchronize(e.stream);
+        if (err != hipSuccess) return restore(set_error(e, CM_ERR_HIP, hipGetErrorString(err)));
+        errs.reserve(nt);
+        for (int t = 0; t < nt; t++) {
+            const int i = targets[t];
+            if (!act[i]) continue;
+            const double x = hx[t], y = hy[t], z = hz[t];
+            const double mag = sqrt(x * x + y * y + z * z);
+            if (mag < 1e-30) continue;
+            const double dx = ax[i] - x, dy = ay[i] - y, dz = az[i] - z;
+            errs.push_back(sqrt(dx * dx + dy * dy + dz * dz) / mag);
+        }
+    }
+    error_statistics(errs, out);
+    out->theta = theta;
+    return restore(CM_OK);
 }
 
 }  // namespace cm

@@ -410,7 +410,9 @@ k_traverse(TravArgs a) {
         // order at the end.  The staged list is stored so that each sub's
         // entries are contiguous: entry j sits in slot (j % kl) * W + j / kl,
         // W = 64 / kl, so lane sub reads slots sub * W + t, t = 0, 1, ...
-        constexpr bool kSplit = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
+        // The FP64 baseline (FP64_FAST) splits small groups the same way.
+        constexpr bool kSplit = (PREC == CM_PREC_FP32_SPLIT && !LEARNED) ||
+                                (PREC == CM_PREC_FP64_FAST && !POT);
         int kl = 1, klog = 0;
         if constexpr (kSplit) {
             while (2 * kl * nmem <= kWave) { kl *= 2; klog++; }
@@ -421,12 +423,17 @@ k_traverse(TravArgs a) {
         const int slot = ((lane & (kl - 1)) << (6 - klog)) | (lane >> klog);
         const int rbase = worker ? sub * kw : 0;
         float txw = txf, tyw = tyf, tzw = tzf;
+        double xw = xi, yw = yi, zw = zi;   // FP64_FAST: the lane's target
         if constexpr (kSplit) {
             if (worker && sub > 0) {
                 const int mt = s + tl;
-                txw = (float)(sx[mt] - cxg);
-                tyw = (float)(sy[mt] - cyg);
-                tzw = (float)(sz[mt] - czg);
+                if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                    txw = (float)(sx[mt] - cxg);
+                    tyw = (float)(sy[mt] - cyg);
+                    tzw = (float)(sz[mt] - czg);
+                } else {
+                    xw = sx[mt]; yw = sy[mt]; zw = sz[mt];
+                }
             }
         }
         double acx = 0.0, acy = 0.0, acz = 0.0;   // monopole sums
@@ -534,6 +541,8 @@ k_traverse(TravArgs a) {
                 // the slots past cnt hold massless far entries, whose terms
                 // are exactly zero, so the loops below may run past cnt
                 if (lane >= cnt) st.v[slot] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+            } else if constexpr (kSplit) {
+                if (lane >= cnt) st.v[slot] = make_double4(1e18, 1e18, 1e18, 0.0);
             }
             if (lane < cnt) {
                 if (cells) {
@@ -542,7 +551,7 @@ k_traverse(TravArgs a) {
                         st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
                                                  (float)(v.z - czg), (float)v.w);
                     } else {
-                        st.v[lane] = v;
+                        st.v[kSplit ? slot : lane] = v;
                     }
                 } else {
                     int q = s_part[lane];
@@ -551,7 +560,7 @@ k_traverse(TravArgs a) {
                                                  (float)(sz[q] - czg), (float)sm[q]);
                         st.r[lane] = shell ? (float)sr[q] : 0.0f;
                     } else {
-                        st.v[lane] = make_double4(sx[q], sy[q], sz[q], sm[q]);
+                        st.v[kSplit ? slot : lane] = make_double4(sx[q], sy[q], sz[q], sm[q]);
                         st.r[lane] = shell ? sr[q] : 0.0;
                     }
                     st.id[lane] = q;
@@ -620,15 +629,46 @@ k_traverse(TravArgs a) {
                         if (cells) acx -= v.w / dist;
                         else apx -= v.w / dist;
                     }
+                } else if constexpr (PREC == CM_PREC_FP64_FAST) {
+                    // the FP64 baseline: FMA-contracted pair terms (it is
+                    // judged against the IEEE path by tolerance, not bit for
+                    // bit), lanes split over small groups as in FP32 split
+                    double gx = 0.0, gy = 0.0, gz = 0.0;
+                    auto term = [&](const double4 v, double f_shell_r) {
+                        const double dx = xw - v.x, dy = yw - v.y, dz = zw - v.z;
+                        double f;
+                        if (f_shell_r < 0.0) {
+                            const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy,
+                                              __builtin_fma(dz, dz, a.eps2)));
+                            const double inv = rsqrt_fast(r2);
+                            f = v.w * (inv * inv * inv);
+                        } else {
+                            f = gk_coeff(dx * dx + dy * dy + dz * dz, v.w, f_shell_r, a.eps2, true);
+                        }
+                        gx = __builtin_fma(-f, dx, gx);
+                        gy = __builtin_fma(-f, dy, gy);
+                        gz = __builtin_fma(-f, dz, gz);
+                    };
+                    const int trips = (cnt + kl - 1) >> klog;
+                    if (cells || (!shell && a.eps2 >= kSelfTermMinEps2)) {
+                        // the self term (dx = 0) is exactly zero at this
+                        // softening; entries past cnt are massless and far
+                        for (int t = 0; t < trips; t++) term(st.v[rbase + t], -1.0);
+                    } else {
+                        const int mt = s + tl;
+                        for (int t = 0, j = sub; j < cnt; t++, j += kl) {
+                            if (st.id[j] == mt) continue;
+                            term(st.v[rbase + t], shell ? st.r[j] : -1.0);
+                        }
+                    }
+                    apx += gx; apy += gy; apz += gz;
                 } else if (cells) {
                     for (int j = 0; j < cnt; j++) {
                         double4 v = st.v[j];
                         double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
                         double dist2 = dx * dx + dy * dy + dz * dz;
                         double soft = dist2 + a.eps2;
-                        double inv;
-                        if constexpr (PREC == CM_PREC_FP64_FAST) inv = rsqrt_fast(soft);
-                        else inv = 1.0 / sqrt(soft);
+                        double inv = 1.0 / sqrt(soft);
                         double inv3 = inv * inv * inv;
                         double f = v.w * inv3;
                         acx -= f * dx; acy -= f * dy; acz -= f * dz;
@@ -639,17 +679,7 @@ k_traverse(TravArgs a) {
                         double4 v = st.v[j];
                         double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
                         double dist2 = dx * dx + dy * dy + dz * dz;
-                        double f;
-                        if constexpr (PREC == CM_PREC_FP64_FAST) {
-                            if (!shell) {
-                                double inv = rsqrt_fast(dist2 + a.eps2);
-                                f = v.w * inv * inv * inv;
-                            } else {
-                                f = gk_coeff(dist2, v.w, st.r[j], a.eps2, true);
-                            }
-                        } else {
-                            f = gk_coeff(dist2, v.w, st.r[j], a.eps2, shell);
-                        }
+                        double f = gk_coeff(dist2, v.w, st.r[j], a.eps2, shell);
                         apx -= f * dx; apy -= f * dy; apz -= f * dz;
                     }
                 }

@@ -83,6 +83,7 @@ internal static unsafe class CmHip
     [DllImport(Lib)] public static extern int cm_configure(IntPtr engine, ref BackendConfig cfg);
     [DllImport(Lib)] public static extern int cm_upload_soa(IntPtr engine, ref SoAPtrs src, int count);
     [DllImport(Lib)] public static extern int cm_download_soa(IntPtr engine, ref SoAPtrs dst, int count);
+    [DllImport(Lib)] public static extern int cm_upload_force_inputs(IntPtr engine, ref SoAPtrs src, int count);
     [DllImport(Lib)] public static extern int cm_compute_forces(IntPtr engine, double softening);
     [DllImport(Lib)] public static extern int cm_verlet_step(IntPtr engine, double softening, double dt, int nSteps);
     [DllImport(Lib)] public static extern int cm_yoshida4_step(IntPtr engine, double softening, double dt, int nSteps);
@@ -177,10 +178,13 @@ public sealed unsafe class HipPhysicsBackend : IPhysicsComputeBackend, IGravityM
 
     private CmHip.PnConfig _pn;
 
+    // allocated once: ComputeForces allocates nothing per call
+    private readonly CmHip.PnWarning[] _warnings = new CmHip.PnWarning[4096];
+
     private void RaiseProximityWarnings()
     {
         if (_pn.Enabled == 0 || OnSchwarzschildProximityWarning == null) return;
-        var buf = new CmHip.PnWarning[4096];
+        var buf = _warnings;
         fixed (CmHip.PnWarning* w = buf)
         {
             CmHip.Check(_engine, CmHip.cm_pn_warnings(_engine, w, buf.Length, out int total), "cm_pn_warnings");
@@ -192,22 +196,23 @@ public sealed unsafe class HipPhysicsBackend : IPhysicsComputeBackend, IGravityM
     public void ComputeForces(BodySoA bodies, double softening)
     {
         if (bodies.Count > _capacity) throw new InvalidOperationException("body count exceeds backend capacity");
+        // only what the force evaluation reads crosses PCIe (positions,
+        // masses, radii, active flags; velocities and body types when the 1PN
+        // decorator or per-type softening reads them), and only AccX/Y/Z back
         fixed (double* px = bodies.PosX, py = bodies.PosY, pz = bodies.PosZ,
                vx = bodies.VelX, vy = bodies.VelY, vz = bodies.VelZ,
                ax = bodies.AccX, ay = bodies.AccY, az = bodies.AccZ,
-               ox = bodies.OldAccX, oy = bodies.OldAccY, oz = bodies.OldAccZ,
-               m = bodies.Mass, r = bodies.Radius, d = bodies.Density)
-        fixed (bool* act = bodies.IsActive, col = bodies.IsCollidable)
+               m = bodies.Mass, r = bodies.Radius)
+        fixed (bool* act = bodies.IsActive)
         fixed (int* type = bodies.BodyTypeIndex)
         {
             var p = new CmHip.SoAPtrs
             {
                 PosX = px, PosY = py, PosZ = pz, VelX = vx, VelY = vy, VelZ = vz,
-                AccX = ax, AccY = ay, AccZ = az, OldAccX = ox, OldAccY = oy, OldAccZ = oz,
-                Mass = m, Radius = r, Density = d,
-                IsActive = (byte*)act, IsCollidable = (byte*)col, BodyTypeIndex = type,
+                Mass = m, Radius = r, IsActive = (byte*)act, BodyTypeIndex = type,
             };
-            CmHip.Check(_engine, CmHip.cm_upload_soa(_engine, ref p, bodies.Count), "cm_upload_soa");
+            CmHip.Check(_engine, CmHip.cm_upload_force_inputs(_engine, ref p, bodies.Count),
+                        "cm_upload_force_inputs");
             CmHip.Check(_engine, CmHip.cm_compute_forces(_engine, softening), "cm_compute_forces");
             var outp = new CmHip.SoAPtrs { AccX = ax, AccY = ay, AccZ = az };
             CmHip.Check(_engine, CmHip.cm_download_soa(_engine, ref outp, bodies.Count), "cm_download_soa");

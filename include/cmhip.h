@@ -159,6 +159,15 @@ int32_t cm_default_config(cm_backend_config* cfg);
 /* BodySoA.CopyFrom-equivalent upload (SoA/BodySoA.cs:118-153), replacing
  * celestial_set_particles (native_api.h:41-47).  count <= capacity. */
 int32_t cm_upload_soa(void* engine, const cm_soa_ptrs* src, int32_t count);
+/* The part of BodySoA that IPhysicsComputeBackend.ComputeForces reads
+ * (Solvers/IPhysicsComputeBackend.cs:21-33: positions, masses, radii, active
+ * flags), for a host that keeps the bodies and calls the backend once per
+ * step: pos_*, mass and is_active are required; radius is required with the
+ * shell theorem (else uploaded when given); vel_* are uploaded only (and then
+ * required) with the 1PN decorator on, body_type only with per-type
+ * softening.  The device's other arrays keep their contents.  41 bytes per
+ * body instead of the 126 of cm_upload_soa.  count <= capacity. */
+int32_t cm_upload_force_inputs(void* engine, const cm_soa_ptrs* src, int32_t count);
 /* BodySoA.CopyTo-equivalent download (SoA/BodySoA.cs:160-174), replacing
  * celestial_get_accelerations/positions/velocities (native_api.h:50-60). */
 int32_t cm_download_soa(void* engine, const cm_soa_ptrs* dst, int32_t count);
@@ -214,6 +223,28 @@ int32_t cm_max_acceleration(void* engine, double* out);
 int32_t cm_direct_sample(void* engine, double softening,
                          const int32_t* targets, int32_t n_targets,
                          double* acc_x, double* acc_y, double* acc_z);
+/* ForceErrorAnalyzer.CompareForces(bodies, softening, theta)
+ * (Validation/ForceErrorAnalyzer.cs:67-135): exact FP64 direct-sum forces,
+ * then the Barnes-Hut backend's forces at `theta` (the engine's configured
+ * precision, leaf capacity, group size and far field; global softening, no
+ * shell theorem, no 1PN, as the analyzer's plain backends), and per active
+ * body the relative error |a_bh - a_exact| / |a_exact|, skipping bodies with
+ * |a_exact| < 1e-30; mean, max and RMS summed in body order, plus the
+ * median (build-added).  n_targets = 0 compares every body (the exact sum is
+ * O(N^2) on the GPU); otherwise only the listed targets, in list order
+ * (exact forces as cm_direct_sample).  Like the reference, the bodies' acc
+ * holds the Barnes-Hut forces afterwards. */
+typedef struct {
+    double mean_relative_error;    /* ComparisonResult.MeanRelativeError */
+    double max_relative_error;     /* MaxRelativeError */
+    double rms_relative_error;     /* RmsRelativeError */
+    double median_relative_error;  /* no reference counterpart */
+    int32_t body_count;            /* BodyCount: bodies compared */
+    double theta;                  /* Theta */
+} cm_force_comparison;
+int32_t cm_compare_forces(void* engine, double softening, double theta,
+                          const int32_t* targets, int32_t n_targets,
+                          cm_force_comparison* out);
 int32_t cm_get_force_stats(void* engine, cm_force_stats* out);
 int32_t cm_reset_force_stats(void* engine);
 
@@ -269,6 +300,38 @@ int32_t cm_resolve_collisions(void* engine, int32_t* n_resolved);
  * *n_resolved (may be NULL) = resolved events summed over the steps. */
 int32_t cm_nbody_step(void* engine, double softening, double dt, int32_t n_steps,
                       int32_t collisions, int32_t* n_resolved);
+/* SimulationState (Types/SimulationState.cs:9-25) as NBodySolver.Step returns
+ * it after each step (Solvers/NBodySolver.cs:307-379).  Energies are
+ * EnergyCalculator's with compensated sums; the potential is the exact pair
+ * sum (with_potential = 1), the Barnes-Hut potential of cm_energy
+ * (with_potential = 2) or 0 (with_potential = 0: kinetic-only totals). */
+typedef struct {
+    double time;                        /* Time: dt summed over the diagnosed steps
+                                           since the last cm_upload_soa */
+    int32_t body_count;                 /* BodyCount */
+    int32_t active_body_count;          /* ActiveBodyCount */
+    double kinetic_energy;              /* KineticEnergy */
+    double potential_energy;            /* PotentialEnergy */
+    double total_energy;                /* TotalEnergy */
+    double total_momentum[3];           /* TotalMomentum */
+    double energy_drift;                /* EnergyDrift: (E - E0) / |E0|, E0 the total
+                                           energy after the first diagnosed step since
+                                           the last upload; 0 when |E0| <= 1e-15 */
+    int32_t collision_count;            /* CollisionCount: resolved this step */
+    double current_dt;                  /* CurrentDt */
+    double step_energy_delta;           /* StepEnergyDelta: E before - E after */
+    double step_momentum_delta;         /* StepMomentumDelta: |P before - P after| */
+    double step_angular_momentum_delta; /* StepAngularMomentumDelta: ||L| before - |L| after| */
+} cm_simulation_state;
+/* n_steps x NBodySolver.Step (Solvers/NBodySolver.cs:307-379): cm_nbody_step's
+ * device-resident step (Verlet with the configured backend, then collisions
+ * when collisions = 1) with the SimulationState of every step in
+ * states[n_steps].  The energy sums of each step are reduced on the device
+ * into one history buffer and read back once at the end (the reference
+ * computes them on the CPU twice per step). */
+int32_t cm_nbody_step_diag(void* engine, double softening, double dt, int32_t n_steps,
+                           int32_t collisions, int32_t with_potential,
+                           cm_simulation_state* states);
 /* Order-preserving removal of inactive bodies, ParticleSystem::compact
  * (engine/src/physics/particle_system.cpp:156-184).  old_index (may be NULL)
  * receives, for each survivor, its index before compaction. */
