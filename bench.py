@@ -525,7 +525,7 @@ def merger(args, cmhip):
         farf = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup,
                         eps, dt, sample, far=args.far_field_ratio)
         log(f"merger theta={theta}: training on a 65,536-body subsample ...")
-        params, tr = train_network(cmhip, soa, theta, eps)
+        params, tr = train_network(cmhip, soa, theta, eps, epochs=args.train_epochs)
         learned = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup,
                            eps, dt, sample, params=params)
         res[str(theta)] = {
@@ -556,6 +556,41 @@ def merger(args, cmhip):
     print(json.dumps(line), flush=True)
 
 
+def only_line(args, cmhip):
+    """One timed line on the headline workload, for profiling runs: the plain
+    FP64 GPU path (theta 0.5) or the learned correction at --theta-learned
+    (network trained first)."""
+    n = args.n or (1 << 20)
+    eps, dt = 1e-4, 1e-3
+    soa = cmhip.generate_plummer(n, 1.0, 1.0, seed=42, mass_fraction_cut=0.999,
+                                 com_frame=True)
+    sample = np.sort(np.random.default_rng(4096).choice(n, min(4096, n),
+                                                         replace=False)).astype(np.int32)
+    if args.only == "fp64":
+        rec = run_path(cmhip, soa, cmhip.PREC_FP64_FAST, 0.5, args.steps, args.warmup, eps, dt,
+                       sample)
+        kernel = "k_traverse<FP64_FAST>"
+        theta = 0.5
+    else:
+        theta = args.theta_learned or 0.75
+        params, tr = train_network(cmhip, soa, theta, eps, epochs=args.train_epochs)
+        rec = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, theta, args.steps, args.warmup, eps,
+                       dt, sample, params=params)
+        kernel = "k_traverse<FP32_SPLIT, learned>"
+    line = {"metric": f"BH body-steps/s at N=1M, {args.only} line only",
+            "value": rec["body_steps_per_s"], "unit": "body-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": rec["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f64" if args.only == "fp64" else "f32-split (f64 state)",
+            "data": "synthetic",
+            "config": {"workload": "plummer_bh_verlet", "n_bodies": n, "theta": theta,
+                       "leaf_capacity": LEAF_CAPACITY, "seed": 42},
+            "kernel": kernel, "ms_kernel_avg": rec["ms_force_kernel_avg"],
+            "interactions_per_body": rec["interactions_per_body"],
+            "force_err_median": rec["force_err_median"]}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -578,6 +613,11 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fast-only", action="store_true",
                     help="time only the FP32 theta = 0.5 line (profiling runs)")
+    ap.add_argument("--train-epochs", type=int, default=20,
+                    help="epochs of the learned correction's training")
+    ap.add_argument("--only", choices=("fast", "fp64", "learned"), default=None,
+                    help="time only one line: the fast path, the FP64 baseline or the "
+                         "learned correction (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run device copies of known size (PMC scale reference)")
     args = ap.parse_args()
@@ -603,6 +643,10 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         extensions(args, cmhip)
         return
+    if args.only == "fast":
+        args.fast_only = True
+    if args.only in ("fp64", "learned"):
+        return only_line(args, cmhip)
     if args.fast_only:
         args.no_fp64 = True
         args.no_cpu_baseline = True
@@ -645,7 +689,7 @@ def main():
     learned = None
     if args.theta_learned:
         log(f"learned correction at theta={args.theta_learned}: training ...")
-        params, tr = train_network(cmhip, soa, args.theta_learned, eps)
+        params, tr = train_network(cmhip, soa, args.theta_learned, eps, epochs=args.train_epochs)
         log(json.dumps(tr))
         learned = run_path(cmhip, soa, cmhip.PREC_FP32_SPLIT, args.theta_learned, args.steps,
                            args.warmup, eps, dt, sample, params=params)
