@@ -324,7 +324,11 @@ k_gather(int n, const int32_t* __restrict__ order, const double4* __restrict__ p
 // k_tree_scan turns the counts into offsets, k_tree_emit writes the nodes and
 // the leaves' sums, k_tree_internal forms internal nodes bottom-up.
 constexpr int kTreeWaves = kTreeTile / kWave;
-constexpr int kTreeHalo = 64;                   // >= largest leaf capacity
+// bodies staged past each tile: >= the largest leaf capacity (the D window of
+// k_tree_count), and large enough that deepest-level leaves, which may hold
+// more bodies than the leaf capacity (up to ~120 in the core of a 2^20-body
+// Plummer sphere with 30-bit keys), are usually summed from LDS
+constexpr int kTreeHalo = 128;
 
 __device__ __forceinline__ int common_digits(uint64_t a, uint64_t b, int levels) {
     uint64_t x = a ^ b;
@@ -554,7 +558,17 @@ k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict_
             end = min(end + stop, lim);
         }
         if (end == lim) {
-            while (end < n && (int)dl[end].x >= l) end++;
+            // past the staged part: eight global loads in flight per round
+            for (bool found = false; !found && end < n;) {
+                int stop = 8;
+#pragma unroll
+                for (int u = 7; u >= 0; u--) {
+                    const int q = end + u;
+                    if (q >= n || (int)dl[q].x < l) stop = u;
+                }
+                found = stop < 8;
+                end += stop;
+            }
         }
         if (end > n) end = n;
         // sums in sorted order, four bodies loaded per round
@@ -572,8 +586,19 @@ k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict_
             for (int u = 0; u < 4; u++)
                 if (j + u < mid) leaf_accumulate(bx[u], by[u], bz[u], bm[u], M, ax, ay, az, lo, hi);
         }
-        for (int j = mid; j < end; j++)
-            leaf_accumulate(sx[j], sy[j], sz[j], sm[j], M, ax, ay, az, lo, hi);
+        // the rest from global memory, four bodies loaded per round (summed
+        // in order, so the sums are the same)
+        for (int j = mid; j < end; j += 4) {
+            double bx[4], by[4], bz[4], bm[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int q = min(j + u, end - 1);
+                bx[u] = sx[q]; by[u] = sy[q]; bz[u] = sz[q]; bm[u] = sm[q];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (j + u < end) leaf_accumulate(bx[u], by[u], bz[u], bm[u], M, ax, ay, az, lo, hi);
+        }
         store_node_sums(k, M, ax, ay, az, lo, hi, cm, sums, box);
         info[k] = make_int4(-1, l << 8, i, end - i);
         if (quad) {

@@ -1426,8 +1426,14 @@ static void merge_sort_events(cmo_event_t* e, cmo_event_t* tmp, int lo, int hi) 
     merge_sort_events(e, tmp, mid, hi);
     int i = lo, j = mid, k = lo;
     while (i < mid && j < hi) {
-        /* descending overlap; equal overlaps keep (i, j) order */
-        if (e[j].overlap > e[i].overlap) tmp[k++] = e[j++];
+        /* descending overlap; equal overlaps in (i, j) order of the pair */
+        int before = e[j].overlap > e[i].overlap;
+        if (e[j].overlap == e[i].overlap) {
+            int ji = e[j].a < e[j].b ? e[j].a : e[j].b, jj = e[j].a < e[j].b ? e[j].b : e[j].a;
+            int ii = e[i].a < e[i].b ? e[i].a : e[i].b, ij = e[i].a < e[i].b ? e[i].b : e[i].a;
+            before = ji < ii || (ji == ii && jj < ij);
+        }
+        if (before) tmp[k++] = e[j++];
         else tmp[k++] = e[i++];
     }
     while (i < mid) tmp[k++] = e[i++];
@@ -1447,21 +1453,40 @@ int cmo_detect(int n, const double* px, const double* py, const double* pz,
                const double* vx, const double* vy, const double* vz,
                const double* m, const double* rad, const uint8_t* act,
                const uint8_t* col, cmo_event_t* out, int cap) {
+    /* every pair i < j is tested (brute force); threads collect the events
+     * of interleaved rows, and the sort below fixes the order, so the result
+     * does not depend on the thread count */
     int ne = 0, ecap = 256;
     cmo_event_t* ev = (cmo_event_t*)malloc((size_t)ecap * sizeof(cmo_event_t));
-    for (int i = 0; i < n; i++) {
-        if (!act[i] || !col[i]) continue;
-        for (int j = i + 1; j < n; j++) {
-            if (!act[j] || !col[j]) continue;
-            cmo_event_t e;
-            if (try_event(i, j, px, py, pz, vx, vy, vz, m, rad, &e)) {
-                if (ne == ecap) {
-                    ecap *= 2;
-                    ev = (cmo_event_t*)realloc(ev, (size_t)ecap * sizeof(cmo_event_t));
+#pragma omp parallel
+    {
+        int lne = 0, lcap = 64;
+        cmo_event_t* loc = (cmo_event_t*)malloc((size_t)lcap * sizeof(cmo_event_t));
+#pragma omp for schedule(dynamic, 64)
+        for (int i = 0; i < n; i++) {
+            if (!act[i] || !col[i]) continue;
+            for (int j = i + 1; j < n; j++) {
+                if (!act[j] || !col[j]) continue;
+                cmo_event_t e;
+                if (try_event(i, j, px, py, pz, vx, vy, vz, m, rad, &e)) {
+                    if (lne == lcap) {
+                        lcap *= 2;
+                        loc = (cmo_event_t*)realloc(loc, (size_t)lcap * sizeof(cmo_event_t));
+                    }
+                    loc[lne++] = e;
                 }
-                ev[ne++] = e;
             }
         }
+#pragma omp critical
+        {
+            if (ne + lne > ecap) {
+                while (ne + lne > ecap) ecap *= 2;
+                ev = (cmo_event_t*)realloc(ev, (size_t)ecap * sizeof(cmo_event_t));
+            }
+            if (lne > 0) memcpy(ev + ne, loc, (size_t)lne * sizeof(cmo_event_t));
+            ne += lne;
+        }
+        free(loc);
     }
     cmo_event_t* tmp = (cmo_event_t*)malloc((size_t)(ne + 1) * sizeof(cmo_event_t));
     merge_sort_events(ev, tmp, 0, ne);

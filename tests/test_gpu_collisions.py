@@ -171,3 +171,38 @@ def test_nbody_step_with_collisions_bit_exact(mode):
     out = cmhip.BodySoA(soa.count)
     be.download(out)
     assert_state_equal(out, s)
+
+
+def test_disk_configuration_cycles_at_full_size():
+    """The collision configuration at its own size: N = 262,144 cold thin disk
+    (1 Msun star, a log-uniform 0.5-5 AU, Rayleigh e 0.02 / i 0.01, masses
+    log-uniform 1e-9 - 1e-6 Msun, radii from 3 g/cm^3 inflated 100x, seed 42,
+    the benchmark's generator); two cycles of Verlet with FP64 Barnes-Hut
+    (theta 0.5), detection, MergeOnly resolution and compaction equal the
+    oracle's sequence bit for bit (event counts, survivor map, every array).
+    The oracle's detection is the brute-force pair test over all 3.4e10 pairs."""
+    n = 262144
+    print(f"disk cycles: n={n}", flush=True)
+    soa = cmhip.generate_disk(n, seed=42, a_min=0.5, a_max=5.0, sigma_e=0.02, sigma_i=0.01,
+                              m_min=1e-9, m_max=1e-6, density=cmhip.ROCK_DENSITY_INFLATED)
+    s = soa_to_state(soa)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5)
+    be.upload(soa)
+    eps, dt = 1e-4, 1e-3
+    for cycle in range(2):
+        be.step(eps, dt, 1)
+        _, ne = be.detect_collisions(0)
+        merged = be.resolve_merges()
+        n_new, old = be.compact()
+        O.verlet_step(s, dt, lambda st: O.bh_tree(st, eps, 0.5, levels=10, leaf_cap=16,
+                                                  group_size=64, nthreads=0)[0])
+        ev = O.detect(s)
+        ref_merged = O.resolve_merge(ev, s)
+        s, idx = O.compact(s)
+        print(f"cycle {cycle}: events {ne}, merged {merged}, bodies {n_new}", flush=True)
+        assert ne == len(ev) and merged == ref_merged and merged > 0
+        assert n_new == len(idx)
+        np.testing.assert_array_equal(old, idx)
+    out = cmhip.BodySoA(n)
+    be.download(out)
+    assert_state_equal(out, s)
