@@ -43,7 +43,7 @@ import numpy as np  # noqa: E402
 PEAK_FP32_VALU_TFLOPS = 157.3   # MI355X published FP32 vector peak
 FLOPS_PER_INTERACTION = 20
 FLOPS_PER_FAR_CELL = 76         # Taylor coefficients of one (group, cell) far-field term
-FAR_FIELD_RATIO = 0.15          # group far-field expansion ratio of the FP32 split path
+FAR_FIELD_RATIO = 0.25          # group far-field expansion ratio of the FP32 split path (swept: 0.15-0.3)
 LEAF_CAPACITY = 8               # bodies per leaf bucket (measured best at N = 2^20, theta 0.5-0.65)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_traverse_plummer1M_traffic.md")
 
@@ -592,6 +592,7 @@ def only_line(args, cmhip):
 
 
 def main():
+    global LEAF_CAPACITY
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -613,6 +614,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--fast-only", action="store_true",
                     help="time only the FP32 theta = 0.5 line (profiling runs)")
+    ap.add_argument("--leaf-capacity", type=int, default=LEAF_CAPACITY,
+                    help="bodies per leaf bucket of the Barnes-Hut lines")
     ap.add_argument("--train-epochs", type=int, default=20,
                     help="epochs of the learned correction's training")
     ap.add_argument("--only", choices=("fast", "fp64", "learned"), default=None,
@@ -621,6 +624,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run device copies of known size (PMC scale reference)")
     args = ap.parse_args()
+    LEAF_CAPACITY = args.leaf_capacity
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)

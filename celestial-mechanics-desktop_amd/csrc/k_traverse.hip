@@ -243,12 +243,30 @@ __device__ __forceinline__ double wmax(double v) {
     }
     return v;
 }
+// Inclusive prefix sum over the 64 lanes of a wavefront (all lanes active):
+// Hillis-Steele within each row of 16 lanes by DPP row shifts of 1, 2, 4 and 8
+// (lanes shifted in from outside the row contribute 0), then the row totals
+// carried forward by the row broadcasts of lanes 15 and 31.  Integer adds, so
+// the result is exact; it replaces six LDS permutes.
 __device__ __forceinline__ int wscan_incl(int v) {
-    int lane = lane_id();
-    for (int o = 1; o < kWave; o <<= 1) {
-        int t = __shfl_up(v, o, kWave);
-        if (lane >= o) v += t;
-    }
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31
+    return v;
+}
+
+// Inclusive running maximum over the 64 lanes (values >= -1; all lanes
+// active), by the same DPP pattern as wscan_incl with -1 shifted in.
+__device__ __forceinline__ int wmax_scan(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(-1, v, 0x143, 0xc, 0xf, false));
     return v;
 }
 
@@ -325,6 +343,7 @@ k_traverse(TravArgs a) {
     __shared__ int s_part[kWave];
     __shared__ int s_lpref[kWave];
     __shared__ int s_lstart[kWave];
+    __shared__ int s_mark[kWave];
     __shared__ int s_g;
     __shared__ Staging<PREC> st;
     // MLP inputs, bf16 [cell][16]; read as 8-byte operands
@@ -767,14 +786,14 @@ k_traverse(TravArgs a) {
             // opened internal nodes push their children, in lane order
             int nop = kind == 3 ? (nd.y & 255) : 0;
             int oin = wscan_incl(nop);
-            int otot = __builtin_amdgcn_readfirstlane(__shfl(oin, kWave - 1, kWave));
+            int otot = __builtin_amdgcn_readlane(oin, kWave - 1);
             if (kind == 3)
                 for (int c = 0; c < nop; c++) s_stack[S + oin - nop + c] = nd.x + c;
             S += otot;
             // opened leaves: body ranges, expanded below in lane order
             int nl = kind == 2 ? nd.w : 0;
             int lin = wscan_incl(nl);
-            int ltot = __builtin_amdgcn_readfirstlane(__shfl(lin, kWave - 1, kWave));
+            int ltot = __builtin_amdgcn_readlane(lin, kWave - 1);
             s_lpref[lane] = lin - nl;
             s_lstart[lane] = nd.z;
             __syncthreads();
@@ -798,19 +817,28 @@ k_traverse(TravArgs a) {
                 ncell = rest;
                 __syncthreads();
             }
-            int done = 0;
+            // position q of the opened leaves' bodies belongs to the last lane
+            // whose leaf starts at or before q: per window of 64 positions,
+            // each leaf marks its first position with its lane, and a running
+            // maximum over the window (with the owner of the position before
+            // it carried in) gives every position its leaf
+            int done = 0, carry = -1;
             while (done < ltot) {
                 int room = kWave - npart;
                 int tk = ltot - done < room ? ltot - done : room;
+                s_mark[lane] = -1;
+                __syncthreads();
+                const int pref = lin - nl;
+                if (nl > 0 && pref >= done && pref < done + kWave) s_mark[pref - done] = lane;
+                __syncthreads();
+                int own = s_mark[lane];
+                if (lane == 0 && own < 0) own = carry;
+                own = wmax_scan(own);
                 if (lane < tk) {
                     int q = done + lane;
-                    int lo = 0, hi = kWave - 1;   // last lane with pref <= q
-                    while (lo < hi) {
-                        int mid = (lo + hi + 1) >> 1;
-                        if (s_lpref[mid] <= q) lo = mid; else hi = mid - 1;
-                    }
-                    s_part[npart + lane] = s_lstart[lo] + (q - s_lpref[lo]);
+                    s_part[npart + lane] = s_lstart[own] + (q - s_lpref[own]);
                 }
+                carry = __builtin_amdgcn_readlane(own, tk - 1);
                 npart += tk;
                 done += tk;
                 __syncthreads();

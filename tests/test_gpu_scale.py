@@ -14,8 +14,8 @@ oracle on the same input:
     (engine/src/cuda/gpu_tree.cu:269-464);
   * one FP64 Barnes-Hut evaluation bit for bit, with equal interaction counts;
   * the benchmark's fast path (FP32 split precision, group far field
-    kappa = 0.15): per-body relative difference median <= 2e-6, max <= 1e-3,
-    equal cell, body and far-cell counts."""
+    kappa = 0.25, and 0.15): per-body relative difference median <= 2e-6,
+    max <= 1e-3, equal cell, body and far-cell counts."""
 import ctypes as C
 
 import numpy as np
@@ -131,20 +131,21 @@ def test_barnes_hut_fp64_bit_exact_at_bench_size():
     be.close()
 
 
-def test_fast_path_matches_oracle_at_bench_size():
-    """The benchmark's fast path (FP32 split, far field kappa = 0.15, theta
+@pytest.mark.parametrize("kappa", [0.25, 0.15])
+def test_fast_path_matches_oracle_at_bench_size(kappa):
+    """The benchmark's fast path (FP32 split, far field kappa = 0.25, theta
     0.5, leaves of 8) against the oracle's FP64 restatement of the same
     lists: median <= 2e-6, max <= 1e-3, equal term counts."""
-    print(f"fp32 split + far field: n={N}", flush=True)
+    print(f"fp32 split + far field {kappa}: n={N}", flush=True)
     soa = _bench_state()
     s = soa_to_state(soa)
     be = cmhip.HipPhysicsBackend(N, precision=cmhip.PREC_FP32_SPLIT, theta=0.5,
                                  morton_levels=10, leaf_capacity=LEAF, group_size=64,
-                                 far_field_ratio=0.15)
+                                 far_field_ratio=kappa)
     be.compute_forces(soa, EPS)
     gs = be.stats()
     ref, st = O.bh_tree(s, EPS, 0.5, levels=10, leaf_cap=LEAF, group_size=64,
-                        split_depth=-3, lifo=1536, nthreads=0, far_ratio=0.15)
+                        split_depth=-3, lifo=1536, nthreads=0, far_ratio=kappa)
     assert gs["far_cells"] == st["far_cells"] > 0
     assert gs["cell_interactions"] == int(st["n_cells"].astype(np.int64).sum())
     assert gs["body_interactions"] == int(st["n_parts"].astype(np.int64).sum())
