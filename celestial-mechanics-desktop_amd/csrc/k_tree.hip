@@ -232,59 +232,106 @@ k_sort_scan(int ntiles, uint32_t* __restrict__ hist, uint32_t* __restrict__ tota
     if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
+// Stable scatter of one tile: every key of the tile is loaded first (one
+// round of global loads instead of one per 256 keys), ranked within its digit
+// in input order (per-wave ballots, then waves and rounds in order), placed in
+// LDS in digit order, and written out from there so that consecutive lanes
+// write consecutive addresses of each digit's run.  Destinations are the
+// stable ones: digit base + earlier tiles' count of the digit + rank.
 __global__ void __launch_bounds__(kSortThreads)
 k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
                const int32_t* __restrict__ vin, const uint32_t* __restrict__ hist,
                const uint32_t* __restrict__ totals,
                uint64_t* __restrict__ kout, int32_t* __restrict__ vout) {
     constexpr int kWaves = kSortThreads / kWave;
+    constexpr int kR = kSortTile / kSortThreads;
     __shared__ uint32_t base[256];
     __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[kWaves][256];
-    int t = threadIdx.x;
-    int lane = t & (kWave - 1);
-    int w = t / kWave;
+    __shared__ uint32_t loff[256];
+    // per-wave digit counts, two buffers by round parity: a round clears its
+    // buffer while slower waves may still read the other one's offsets
+    __shared__ uint32_t wcnt[2][kWaves][256];
+    __shared__ uint64_t s_key[kSortTile];
+    __shared__ int32_t s_val[kSortTile];
+    const int t = threadIdx.x;
+    const int lane = t & (kWave - 1);
+    const int w = t / kWave;
+    const int tile0 = blockIdx.x * kSortTile;
+    const int cnt = min(kSortTile, n - tile0);
+    uint64_t key[kR];
+    int32_t val[kR];
+#pragma unroll
+    for (int r = 0; r < kR; r++) {
+        const int i = tile0 + r * kSortThreads + t;
+        key[r] = i < n ? kin[i] : 0ull;
+        val[r] = i < n ? vin[i] : 0;
+    }
     {   // digit base = all smaller digits' totals + this digit in earlier tiles
         uint32_t tot;
         uint32_t dbase = block_excl_scan(totals[t], run, &tot);
         base[t] = dbase + hist[(size_t)t * ntiles + blockIdx.x];
     }
     run[t] = 0u;
-    uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    int tile0 = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortTile / kSortThreads; r++) {
-        for (int q = 0; q < kWaves; q++) wcnt[q][t] = 0u;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    uint32_t lrank[kR];
+#pragma unroll
+    for (int r = 0; r < kR; r++) {
+        uint32_t (*wc)[256] = wcnt[r & 1];
+#pragma unroll
+        for (int q = 0; q < kWaves; q++) wc[q][t] = 0u;
         __syncthreads();
-        int i = tile0 + r * kSortThreads + t;
-        bool valid = i < n;
-        uint64_t key = valid ? kin[i] : 0ull;
-        int32_t val = valid ? vin[i] : 0;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
+        const bool valid = r * kSortThreads + t < cnt;
+        const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
+#pragma unroll
         for (int b = 0; b < 8; b++) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = __ballot(valid && bit);
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(valid && bit);
             peers &= bit ? bal : ~bal;
         }
-        uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+        const uint32_t rank = (uint32_t)__popcll(peers & lt);
+        if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
         __syncthreads();
-        {   // thread t owns digit t: offsets of each wave within the round
+        {   // thread t owns digit t: offsets of each wave within the tile
             uint32_t acc = run[t];
+#pragma unroll
             for (int q = 0; q < kWaves; q++) {
-                uint32_t c = wcnt[q][t];
-                wcnt[q][t] = acc;
+                const uint32_t c = wc[q][t];
+                wc[q][t] = acc;
                 acc += c;
             }
             run[t] = acc;
         }
         __syncthreads();
-        if (valid) {
-            uint32_t pos = base[d] + wcnt[w][d] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
+        lrank[r] = wc[w][d] + rank;
+    }
+    {   // run[t] is now the tile's count of digit t (loff is the scan's
+        // scratch until its last barrier)
+        uint32_t tot;
+        const uint32_t o = block_excl_scan(run[t], loff, &tot);
+        loff[t] = o;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kR; r++) {
+        if (r * kSortThreads + t < cnt) {
+            const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
+            const uint32_t slot = loff[d] + lrank[r];
+            s_key[slot] = key[r];
+            s_val[slot] = val[r];
         }
-        __syncthreads();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kR; r++) {
+        const int q = r * kSortThreads + t;
+        if (q < cnt) {
+            const uint64_t k = s_key[q];
+            const uint32_t d = (uint32_t)(k >> shift) & 255u;
+            const uint32_t pos = base[d] + (uint32_t)q - loff[d];
+            kout[pos] = k;
+            vout[pos] = s_val[q];
+        }
     }
 }
 
@@ -442,6 +489,11 @@ k_tree_scan(int levels, int cap, int ntiles, int tstride, int node_cap,
     }
 }
 
+// The mass and moment sums run in sorted order (bit-identical to the CPU
+// restatement); the box is a min / max, which no order changes, taken with
+// v_min_f64 / v_max_f64 (one instruction each instead of a compare and two
+// selects; they can differ from the compare only in the sign of a zero,
+// which changes no box extent or volume computed from it).
 __device__ __forceinline__ void leaf_accumulate(double x, double y, double z, double mb,
                                                 double& M, double& ax, double& ay,
                                                 double& az, double lo[3], double hi[3]) {
@@ -449,8 +501,8 @@ __device__ __forceinline__ void leaf_accumulate(double x, double y, double z, do
     ax += mb * x;
     ay += mb * y;
     az += mb * z;
-    lo[0] = x < lo[0] ? x : lo[0]; lo[1] = y < lo[1] ? y : lo[1]; lo[2] = z < lo[2] ? z : lo[2];
-    hi[0] = x > hi[0] ? x : hi[0]; hi[1] = y > hi[1] ? y : hi[1]; hi[2] = z > hi[2] ? z : hi[2];
+    lo[0] = fmin(x, lo[0]); lo[1] = fmin(y, lo[1]); lo[2] = fmin(z, lo[2]);
+    hi[0] = fmax(x, hi[0]); hi[1] = fmax(y, hi[1]); hi[2] = fmax(z, hi[2]);
 }
 
 __device__ __forceinline__ void store_node_sums(int k, double M, double ax, double ay,
@@ -472,7 +524,13 @@ __device__ __forceinline__ void store_node_sums(int k, double M, double ax, doub
 // Writes every node below the root: start, parent, first child (internal
 // nodes; their count, child count and sums come from k_tree_internal), and
 // for leaves the count, mass, sum m x, centre of mass and body box, summed
-// in sorted order from the tile's bodies staged in LDS.
+// in sorted order.  Two phases per tile: every body writes the nodes it
+// opens and, when it opens a leaf, queues it (in order of its first body);
+// then the tile's leaves are taken one per thread, so that a wave sums up to
+// 64 leaves instead of the few that start among its 64 bodies (deepest-level
+// leaves hold up to ~120 bodies in the core of a 2^20-body Plummer sphere,
+// and a wave runs as long as its longest leaf).  The bodies of the tile and
+// a halo past it are staged in LDS.
 __global__ void __launch_bounds__(kTreeTile)
 k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict__ hdr,
             const uchar2* __restrict__ dl, const uint32_t* __restrict__ toff,
@@ -486,6 +544,9 @@ k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict_
     __shared__ int s_ls[kMaxLevels + 3];
     __shared__ double s_x[kS], s_y[kS], s_z[kS], s_m[kS];
     __shared__ unsigned char s_d[kS];
+    __shared__ int s_jk[kTreeTile];   // leaf jobs: node index
+    __shared__ int s_jb[kTreeTile];   //            first body - t0, level << 16
+    __shared__ int s_jw[kTreeWaves];  // jobs queued by earlier waves
     if (hdr->overflow) return;
     const int n = hdr->n_active;
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
@@ -523,30 +584,50 @@ k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict_
         s_woff[l][ww] = s;
     }
     __syncthreads();
-    if (i >= n || lam <= d) return;
     const uint64_t below = (1ull << lane) - 1ull;
     auto index_at = [&](int l) -> int {
         return s_ls[l] + (int)s_woff[l][w] + __popcll(s_mask[l][w] & below);
     };
-    const double big = 1.7976931348623157e308;
-    for (int l = d + 1; l <= lam; l++) {
-        int k = index_at(l);
-        if (k >= node_cap) break;
-        // the level l - 1 node holding body i: opened by i itself when
-        // d < l - 1, else by the last body before i that opened one
-        parent[k] = l == 1 ? 0 : index_at(l - 1) + (d < l - 1 ? 1 : 0) - 1;
-        if (l < lam) {
-            info[k] = make_int4(index_at(l + 1), l << 8, i, 0);
-            continue;
+    int leaf_k = -1;
+    if (i < n) {
+        for (int l = d + 1; l <= lam; l++) {
+            int k = index_at(l);
+            if (k >= node_cap) break;
+            // the level l - 1 node holding body i: opened by i itself when
+            // d < l - 1, else by the last body before i that opened one
+            parent[k] = l == 1 ? 0 : index_at(l - 1) + (d < l - 1 ? 1 : 0) - 1;
+            if (l < lam) info[k] = make_int4(index_at(l + 1), l << 8, i, 0);
+            else leaf_k = k;
         }
-        // leaf: bodies i .. end - 1, end = first j > i that opens a level-l
+    }
+    {   // queue the leaf jobs in order of their first body
+        const uint64_t has = __ballot(leaf_k >= 0);
+        if (lane == 0) s_jw[w] = __popcll(has);
+        __syncthreads();
+        int off = 0;
+        for (int q = 0; q < w; q++) off += s_jw[q];
+        if (leaf_k >= 0) {
+            const int jdx = off + __popcll(has & below);
+            s_jk[jdx] = leaf_k;
+            s_jb[jdx] = (i - t0) | (lam << 16);
+        }
+    }
+    __syncthreads();
+    int njobs = 0;
+    for (int q = 0; q < kTreeWaves; q++) njobs += s_jw[q];
+    const double big = 1.7976931348623157e308;
+    const int lim = t0 + kS;
+    for (int job = t; job < njobs; job += kTreeTile) {
+        const int k = s_jk[job];
+        const int b0 = t0 + (s_jb[job] & 0xffff);
+        const int l = s_jb[job] >> 16;
+        // leaf: bodies b0 .. end - 1, end = first j > b0 that opens a level-l
         // cell (deepest-level leaves may hold more than cap bodies); the
-        // candidates are read 8 at a time so their loads overlap
-        // staged part first (s_d is 0 past the last body, which stops the
-        // search), then, for a deepest-level leaf running past the halo, the
-        // rest from global memory
-        const int lim = t0 + kS;
-        int end = i + 1;
+        // candidates are read 8 at a time so their loads overlap: the staged
+        // part first (s_d is 0 past the last body, which stops the search),
+        // then, for a deepest-level leaf running past the halo, the rest from
+        // global memory
+        int end = b0 + 1;
         for (bool found = false; !found && end < lim;) {
             int stop = 8;
 #pragma unroll
@@ -575,7 +656,7 @@ k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict_
         double M = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
         double lo[3] = {big, big, big}, hi[3] = {-big, -big, -big};
         const int mid = min(end, lim);
-        for (int j = i; j < mid; j += 4) {
+        for (int j = b0; j < mid; j += 4) {
             double bx[4], by[4], bz[4], bm[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -600,14 +681,14 @@ k_tree_emit(int levels, int node_cap, int tstride, const TreeHeader* __restrict_
                 if (j + u < end) leaf_accumulate(bx[u], by[u], bz[u], bm[u], M, ax, ay, az, lo, hi);
         }
         store_node_sums(k, M, ax, ay, az, lo, hi, cm, sums, box);
-        info[k] = make_int4(-1, l << 8, i, end - i);
+        info[k] = make_int4(-1, l << 8, b0, end - b0);
         if (quad) {
             // second moments about the centre of mass (learned far-field
             // correction inputs)
             double c0 = 0.0, c1 = 0.0, c2 = 0.0;
             if (M > 0.0) { c0 = ax / M; c1 = ay / M; c2 = az / M; }
             double Q[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-            for (int j = i; j < end; j++) {
+            for (int j = b0; j < end; j++) {
                 int q = j - t0;
                 bool st = j < mid;
                 double dx = (st ? s_x[q] : sx[j]) - c0;
@@ -698,9 +779,8 @@ k_tree_internal(int L, const TreeHeader* __restrict__ hdr, int4* __restrict__ in
             for (int r = 0; r < 6; r++) vb[r] = __shfl(b[r], src, kWave);
             if (q < nch) {
                 M += vm; ax += v0; ay += v1; az += v2; count += vc;
-                lo[0] = vb[0] < lo[0] ? vb[0] : lo[0]; lo[1] = vb[1] < lo[1] ? vb[1] : lo[1];
-                lo[2] = vb[2] < lo[2] ? vb[2] : lo[2]; hi[0] = vb[3] > hi[0] ? vb[3] : hi[0];
-                hi[1] = vb[4] > hi[1] ? vb[4] : hi[1]; hi[2] = vb[5] > hi[2] ? vb[5] : hi[2];
+                lo[0] = fmin(vb[0], lo[0]); lo[1] = fmin(vb[1], lo[1]); lo[2] = fmin(vb[2], lo[2]);
+                hi[0] = fmax(vb[3], hi[0]); hi[1] = fmax(vb[4], hi[1]); hi[2] = fmax(vb[5], hi[2]);
             }
         }
         if (quad) {
