@@ -172,7 +172,10 @@ int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne)
 int collide_resolve(Engine& e, const cm_collision_config& cfg, int32_t* merged);
 int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index);
 void collide_free(Engine& e);
-int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out);
+// Sorted Morton keys are 32-bit words when 3 * levels + 1 bits fit (levels
+// <= 10), else 64-bit.
+inline bool tree_keys32(const Engine& e) { return e.cfg.morton_levels <= 10; }
+int build_tree(Engine& e, const void** keys_out, const int32_t** order_out);
 int form_groups(Engine& e);
 void mlp_free(Engine& e);
 int launch_pn(Engine& e, double softening);

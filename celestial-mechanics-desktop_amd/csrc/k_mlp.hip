@@ -670,7 +670,7 @@ int32_t cm_mlp_collect(void* h, double softening, int32_t max_groups, uint64_t s
     // tree with second moments, target groups
     const int saved = e->cfg.learned_correction;
     e->cfg.learned_correction = 1;
-    const uint64_t* keys = nullptr;
+    const void* keys = nullptr;
     const int32_t* order = nullptr;
     rc = cm::build_tree(*e, &keys, &order);
     e->cfg.learned_correction = saved;

@@ -954,7 +954,7 @@ int launch_barnes_hut(Engine& e, double softening) {
     hipStream_t st = e.stream;
     if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[0], st));
     e.mark(0);
-    const uint64_t* keys = nullptr;
+    const void* keys = nullptr;
     const int32_t* order = nullptr;
     const bool learned = e.cfg.learned_correction != 0;
     if (learned && (e.cfg.precision != CM_PREC_FP32_SPLIT || !mlp_ready(e)))
@@ -1024,7 +1024,7 @@ int launch_barnes_hut(Engine& e, double softening) {
 int launch_bh_potential(Engine& e, double softening, double* rows) {
     const int n = e.count;
     if (n == 0) return CM_OK;
-    const uint64_t* keys = nullptr;
+    const void* keys = nullptr;
     const int32_t* order = nullptr;
     int rc = build_tree(e, &keys, &order);
     if (rc != CM_OK) return rc;

@@ -151,11 +151,14 @@ __device__ __forceinline__ uint32_t quantise(double x, double c, double h,
     return (uint32_t)u;
 }
 
+// Keys are stored as K: 32-bit words when 3 levels + 1 sentinel bit fit
+// (levels <= 10), else 64-bit.
+template <typename K>
 __global__ void __launch_bounds__(256)
 k_morton(int n, int levels, const TreeHeader* __restrict__ hdr,
          const double* __restrict__ px, const double* __restrict__ py,
          const double* __restrict__ pz, const double* __restrict__ m,
-         const uint8_t* __restrict__ act, uint64_t* __restrict__ keys,
+         const uint8_t* __restrict__ act, K* __restrict__ keys,
          int32_t* __restrict__ vals, double4* __restrict__ pm4,
          int32_t* __restrict__ gmark) {
     int i = blockIdx.x * 256 + threadIdx.x;
@@ -171,7 +174,7 @@ k_morton(int n, int levels, const TreeHeader* __restrict__ hdr,
         uint32_t qz = quantise(z, hdr->box[2], h, levels);
         k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
     }
-    keys[i] = k;
+    keys[i] = (K)k;
     vals[i] = i;
     // position and mass packed per body, so that the gather after the sort
     // reads one 32-byte record per body instead of four scattered words
@@ -179,37 +182,62 @@ k_morton(int n, int levels, const TreeHeader* __restrict__ hdr,
 }
 
 // ---------------------------------------------------------- radix sort ----
+template <typename K>
 __global__ void __launch_bounds__(kSortThreads)
-k_sort_hist(int n, int shift, int ntiles, const uint64_t* __restrict__ keys,
+k_sort_hist(int n, int shift, int ntiles, const K* __restrict__ keys,
             uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0u;
     __syncthreads();
-    int base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortTile / kSortThreads; r++) {
-        int i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    const int base = blockIdx.x * kSortTile;
+    constexpr int kR = kSortTile / kSortThreads;
+    K k[kR];   // all loads of the tile in flight before the first count
+#pragma unroll
+    for (int r = 0; r < kR; r++) {
+        const int i = base + r * kSortThreads + threadIdx.x;
+        k[r] = i < n ? keys[i] : (K)0;
     }
+#pragma unroll
+    for (int r = 0; r < kR; r++)
+        if (base + r * kSortThreads + (int)threadIdx.x < n)
+            atomicAdd(&h[(uint32_t)(k[r] >> shift) & 255u], 1u);
     __syncthreads();
     hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
 }
 
-// Block-wide exclusive scan of one value per thread (kSortThreads threads).
+// Inclusive prefix sum over the 64 lanes of a wave (all lanes active): DPP
+// row shifts by 1, 2, 4, 8 within each row of 16 lanes, then the broadcasts
+// of lane 15 into row 1 and 3 and of lane 31 into rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    int x = (int)v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);
+    return (uint32_t)x;
+}
+
+// Block-wide exclusive scan of one value per thread (kSortThreads threads,
+// all active); s_tmp holds one word per wave.
 __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_tmp,
                                                     uint32_t* total) {
-    int t = threadIdx.x;
-    s_tmp[t] = v;
+    constexpr int kWaves = kSortThreads / kWave;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const uint32_t incl = wave_incl_scan(v);
+    if (lane == kWave - 1) s_tmp[w] = incl;
     __syncthreads();
-    for (int o = 1; o < kSortThreads; o <<= 1) {
-        uint32_t x = t >= o ? s_tmp[t - o] : 0u;
-        __syncthreads();
-        s_tmp[t] += x;
-        __syncthreads();
+    uint32_t off = 0u, tot = 0u;
+#pragma unroll
+    for (int q = 0; q < kWaves; q++) {
+        const uint32_t x = s_tmp[q];
+        off += q < w ? x : 0u;
+        tot += x;
     }
-    uint32_t incl = s_tmp[t];
-    *total = s_tmp[kSortThreads - 1];
+    *total = tot;
     __syncthreads();
-    return incl - v;
+    return off + incl - v;
 }
 
 // One block per digit: exclusive scan of that digit's per-tile counts along
@@ -232,55 +260,56 @@ k_sort_scan(int ntiles, uint32_t* __restrict__ hist, uint32_t* __restrict__ tota
     if (threadIdx.x == 0) totals[blockIdx.x] = tot;
 }
 
-// Stable scatter of one tile: every key of the tile is loaded first (one
-// round of global loads instead of one per 256 keys), ranked within its digit
-// in input order (per-wave ballots, then waves and rounds in order), placed in
-// LDS in digit order, and written out from there so that consecutive lanes
-// write consecutive addresses of each digit's run.  Destinations are the
-// stable ones: digit base + earlier tiles' count of the digit + rank.
+// Stable scatter of one tile.  Wave w owns the contiguous tile positions
+// w * 512 .. w * 512 + 511 and ranks them round by round (64 keys per round)
+// against digit counters private to the wave, so the ranking needs no
+// block barrier: within a round, lanes with equal digits find each other by
+// eight ballots and the lowest of them advances the wave's counter.  One
+// barrier later every digit's wave offsets and the tile's digit offsets are
+// known; the keys are placed in LDS in digit order and written out from
+// there, so that consecutive lanes write consecutive addresses of each
+// digit's run.  Destination = digit base + earlier tiles' count of the digit
+// + rank in input order.
+template <typename K>
 __global__ void __launch_bounds__(kSortThreads)
-k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
+k_sort_scatter(int n, int shift, int ntiles, const K* __restrict__ kin,
                const int32_t* __restrict__ vin, const uint32_t* __restrict__ hist,
-               const uint32_t* __restrict__ totals,
-               uint64_t* __restrict__ kout, int32_t* __restrict__ vout) {
+               const uint32_t* __restrict__ totals, K* __restrict__ kout,
+               int32_t* __restrict__ vout) {
     constexpr int kWaves = kSortThreads / kWave;
-    constexpr int kR = kSortTile / kSortThreads;
+    constexpr int kR = kSortTile / kSortThreads;   // keys per lane
+    constexpr int kSpan = kSortTile / kWaves;      // tile positions per wave
     __shared__ uint32_t base[256];
-    __shared__ uint32_t run[256];
     __shared__ uint32_t loff[256];
-    // per-wave digit counts, two buffers by round parity: a round clears its
-    // buffer while slower waves may still read the other one's offsets
-    __shared__ uint32_t wcnt[2][kWaves][256];
-    __shared__ uint64_t s_key[kSortTile];
+    __shared__ uint32_t s_tmp[kWaves];
+    __shared__ uint32_t wcnt[kWaves][256];   // each wave's digit counts, then offsets
+    __shared__ K s_key[kSortTile];
     __shared__ int32_t s_val[kSortTile];
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1);
     const int w = t / kWave;
     const int tile0 = blockIdx.x * kSortTile;
     const int cnt = min(kSortTile, n - tile0);
-    uint64_t key[kR];
+    const int p0 = w * kSpan;
+    K key[kR];
     int32_t val[kR];
 #pragma unroll
     for (int r = 0; r < kR; r++) {
-        const int i = tile0 + r * kSortThreads + t;
-        key[r] = i < n ? kin[i] : 0ull;
-        val[r] = i < n ? vin[i] : 0;
+        const int p = p0 + r * kWave + lane;
+        key[r] = p < cnt ? kin[tile0 + p] : (K)0;
+        val[r] = p < cnt ? vin[tile0 + p] : 0;
     }
+    for (int q = lane; q < 256; q += kWave) wcnt[w][q] = 0u;
     {   // digit base = all smaller digits' totals + this digit in earlier tiles
         uint32_t tot;
-        uint32_t dbase = block_excl_scan(totals[t], run, &tot);
+        const uint32_t dbase = block_excl_scan(totals[t], s_tmp, &tot);
         base[t] = dbase + hist[(size_t)t * ntiles + blockIdx.x];
     }
-    run[t] = 0u;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     uint32_t lrank[kR];
 #pragma unroll
     for (int r = 0; r < kR; r++) {
-        uint32_t (*wc)[256] = wcnt[r & 1];
-#pragma unroll
-        for (int q = 0; q < kWaves; q++) wc[q][t] = 0u;
-        __syncthreads();
-        const bool valid = r * kSortThreads + t < cnt;
+        const bool valid = p0 + r * kWave + lane < cnt;
         const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -290,33 +319,34 @@ k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
             peers &= bit ? bal : ~bal;
         }
         const uint32_t rank = (uint32_t)__popcll(peers & lt);
-        if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        {   // thread t owns digit t: offsets of each wave within the tile
-            uint32_t acc = run[t];
-#pragma unroll
-            for (int q = 0; q < kWaves; q++) {
-                const uint32_t c = wc[q][t];
-                wc[q][t] = acc;
-                acc += c;
-            }
-            run[t] = acc;
+        // the lowest lane of each digit reads and advances the wave's counter
+        uint32_t before = 0u;
+        if (valid && rank == 0) {
+            before = wcnt[w][d];
+            wcnt[w][d] = before + (uint32_t)__popcll(peers);
         }
-        __syncthreads();
-        lrank[r] = wc[w][d] + rank;
+        const int lead = (int)__builtin_ctzll(peers | (1ull << 63));
+        lrank[r] = __shfl(before, lead, kWave) + rank;
     }
-    {   // run[t] is now the tile's count of digit t (loff is the scan's
-        // scratch until its last barrier)
+    __syncthreads();
+    uint32_t dcount = 0u;   // thread t owns digit t: wave offsets, tile count
+#pragma unroll
+    for (int q = 0; q < kWaves; q++) {
+        const uint32_t c = wcnt[q][t];
+        wcnt[q][t] = dcount;
+        dcount += c;
+    }
+    {
         uint32_t tot;
-        const uint32_t o = block_excl_scan(run[t], loff, &tot);
+        const uint32_t o = block_excl_scan(dcount, s_tmp, &tot);
         loff[t] = o;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kR; r++) {
-        if (r * kSortThreads + t < cnt) {
+        if (p0 + r * kWave + lane < cnt) {
             const uint32_t d = (uint32_t)(key[r] >> shift) & 255u;
-            const uint32_t slot = loff[d] + lrank[r];
+            const uint32_t slot = loff[d] + wcnt[w][d] + lrank[r];
             s_key[slot] = key[r];
             s_val[slot] = val[r];
         }
@@ -326,7 +356,7 @@ k_sort_scatter(int n, int shift, int ntiles, const uint64_t* __restrict__ kin,
     for (int r = 0; r < kR; r++) {
         const int q = r * kSortThreads + t;
         if (q < cnt) {
-            const uint64_t k = s_key[q];
+            const K k = s_key[q];
             const uint32_t d = (uint32_t)(k >> shift) & 255u;
             const uint32_t pos = base[d] + (uint32_t)q - loff[d];
             kout[pos] = k;
@@ -384,9 +414,10 @@ __device__ __forceinline__ int common_digits(uint64_t a, uint64_t b, int levels)
     return (3 * levels - 1 - p) / 3;
 }
 
+template <typename K>
 __global__ void __launch_bounds__(kTreeTile)
 k_tree_count(int levels, int cap, int tstride, const TreeHeader* __restrict__ hdr,
-             const uint64_t* __restrict__ keys, uchar2* __restrict__ dl,
+             const K* __restrict__ keys, uchar2* __restrict__ dl,
              uint32_t* __restrict__ tcount) {
     __shared__ signed char sD[kTreeTile + kTreeHalo];
     __shared__ uint32_t s_cnt[kMaxLevels + 1][kTreeWaves];
@@ -815,28 +846,62 @@ k_tree_internal(int L, const TreeHeader* __restrict__ hdr, int4* __restrict__ in
 // first_pass .. first_pass + passes - 1.  On return *k/*v hold the sorted
 // pairs (the buffers are swapped after every pass); hist holds
 // 256 * (ceil(n / kSortTile) + 1) words.
-void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
-                      uint64_t** k, int32_t** v, uint64_t** kt, int32_t** vt,
-                      uint32_t* hist) {
+template <typename K>
+static void sort_pairs(hipStream_t st, int n, int first_pass, int passes, K** k,
+                       int32_t** v, K** kt, int32_t** vt, uint32_t* hist) {
     if (n <= 0) return;
     int ntiles = (n + kSortTile - 1) / kSortTile;
     uint32_t* totals = hist + (size_t)256 * ntiles;
     for (int p = first_pass; p < first_pass + passes; p++) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_sort_hist, dim3(ntiles), dim3(kSortThreads), 0, st, n,
+        hipLaunchKernelGGL(k_sort_hist<K>, dim3(ntiles), dim3(kSortThreads), 0, st, n,
                            shift, ntiles, *k, hist);
         hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kSortThreads), 0, st, ntiles,
                            hist, totals);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(ntiles), dim3(kSortThreads), 0, st,
+        hipLaunchKernelGGL(k_sort_scatter<K>, dim3(ntiles), dim3(kSortThreads), 0, st,
                            n, shift, ntiles, *k, *v, hist, totals, *kt, *vt);
         std::swap(*k, *kt);
         std::swap(*v, *vt);
     }
 }
 
-// Builds the tree for the current state; leaves keys/order in keys_a/vals_a
-// (or keys_b/vals_b when the pass count is odd; *keys_out / *order_out).
-int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) {
+void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
+                      uint64_t** k, int32_t** v, uint64_t** kt, int32_t** vt,
+                      uint32_t* hist) {
+    sort_pairs<uint64_t>(st, n, first_pass, passes, k, v, kt, vt, hist);
+}
+
+// Keys, sort and node counts of the build for K-wide keys.
+template <typename K>
+static int build_keys_and_counts(Engine& e, int n, int levels, const int32_t** order,
+                                 const K** keys) {
+    hipStream_t st = e.stream;
+    const int gb = (n + 255) / 256;
+    K* ka = reinterpret_cast<K*>(e.keys_a);
+    K* kb = reinterpret_cast<K*>(e.keys_b);
+    hipLaunchKernelGGL(k_morton<K>, dim3(gb), dim3(256), 0, st, n, levels, e.hdr,
+                       e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, ka, e.vals_a,
+                       e.pm4, e.gmark);
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
+    const int passes = (3 * levels + 1 + 7) / 8;
+    int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
+    sort_pairs<K>(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist);
+    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
+    // source radii enter only the shell theorem's interior branch
+    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4,
+                       e.cfg.shell_theorem ? e.f[RAD] : nullptr, e.sorted, e.capacity);
+    const int ntiles = std::max(1, (n + kTreeTile - 1) / kTreeTile);
+    hipLaunchKernelGGL(k_tree_count<K>, dim3(ntiles), dim3(kTreeTile), 0, st, levels,
+                       e.cfg.leaf_capacity, tree_tiles(e.capacity), e.hdr, ka, e.tree_dl,
+                       e.tree_tcount);
+    *order = va;
+    *keys = ka;
+    return CM_OK;
+}
+
+// Builds the tree for the current state; *keys_out / *order_out are the sorted
+// keys (32-bit words when tree_keys32(e), else 64-bit) and body order.
+int build_tree(Engine& e, const void** keys_out, const int32_t** order_out) {
     const int n = e.count;
     const int levels = e.cfg.morton_levels;
     hipStream_t st = e.stream;
@@ -845,30 +910,28 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
                        e.f[PZ], e.act, e.bbox_part, e.bbox_cnt);
     hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(1024), 0, st, nb, e.bbox_part,
                        e.bbox_cnt, e.hdr, e.nd_info);
-    int gb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_morton, dim3(gb), dim3(256), 0, st, n, levels, e.hdr,
-                       e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, e.keys_a, e.vals_a,
-                       e.pm4, e.gmark);
-    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
-    int passes = (3 * levels + 1 + 7) / 8;
-    uint64_t* ka = e.keys_a; uint64_t* kb = e.keys_b;
-    int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
-    radix_sort_pairs(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist);
-    if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
-    // source radii enter only the shell theorem's interior branch
-    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4,
-                       e.cfg.shell_theorem ? e.f[RAD] : nullptr, e.sorted, e.capacity);
+    const int32_t* va = nullptr;
+    int rc;
+    if (tree_keys32(e)) {
+        const uint32_t* k32 = nullptr;
+        rc = build_keys_and_counts<uint32_t>(e, n, levels, &va, &k32);
+        *keys_out = k32;
+    } else {
+        const uint64_t* k64 = nullptr;
+        rc = build_keys_and_counts<uint64_t>(e, n, levels, &va, &k64);
+        *keys_out = k64;
+    }
+    if (rc != CM_OK) return rc;
     const int cap = e.cfg.leaf_capacity;
     const int ntiles = std::max(1, (n + kTreeTile - 1) / kTreeTile);
     const int tstride = tree_tiles(e.capacity);
-    hipLaunchKernelGGL(k_tree_count, dim3(ntiles), dim3(kTreeTile), 0, st, levels, cap,
-                       tstride, e.hdr, ka, e.tree_dl, e.tree_tcount);
     hipLaunchKernelGGL(k_tree_scan, dim3(1), dim3(1024), 0, st, levels, cap, ntiles,
                        tstride, e.node_cap, e.hdr, e.tree_tcount, e.nd_info);
     hipLaunchKernelGGL(k_tree_emit, dim3(ntiles), dim3(kTreeTile), 0, st, levels,
                        e.node_cap, tstride, e.hdr, e.tree_dl, e.tree_tcount, e.sorted,
                        e.capacity, e.nd_info, e.nd_parent, e.nd_cm, e.nd_sum, e.nd_box,
                        e.cfg.learned_correction ? e.nd_quad : nullptr);
+    // deepest internal nodes live at level levels - 1
     // deepest internal nodes live at level levels - 1
     for (int L = levels - 1; L >= 0; L--) {
         double maxsz = std::min((double)n, L < 20 ? std::ldexp(1.0, 3 * L) : 1e18);
@@ -878,7 +941,6 @@ int build_tree(Engine& e, const uint64_t** keys_out, const int32_t** order_out) 
                            e.nd_sum, e.nd_box, e.cfg.learned_correction ? e.nd_quad : nullptr);
     }
     CM_HIP_TRY(e, hipGetLastError());
-    *keys_out = ka;
     *order_out = va;
     return CM_OK;
 }
@@ -901,7 +963,7 @@ extern "C" int32_t cm_test_tree(void* h, int32_t node_cap, uint64_t* keys, int32
         if (n_nodes) *n_nodes = 0;
         return CM_OK;
     }
-    const uint64_t* k = nullptr;
+    const void* k = nullptr;
     const int32_t* o = nullptr;
     int rc = build_tree(*e, &k, &o);
     if (rc != CM_OK) return rc;
@@ -913,8 +975,14 @@ extern "C" int32_t cm_test_tree(void* h, int32_t node_cap, uint64_t* keys, int32
     if (n_nodes) *n_nodes = hh.n_nodes;
     if (box4) for (int q = 0; q < 4; q++) box4[q] = hh.box[q];
     if (level_start) for (int q = 0; q < kMaxLevels + 3; q++) level_start[q] = hh.level_start[q];
-    if (keys)
+    std::vector<uint32_t> k32;
+    if (keys && tree_keys32(*e)) {
+        k32.resize((size_t)n);
+        CM_HIP_TRY(*e, hipMemcpyAsync(k32.data(), k, (size_t)n * 4, hipMemcpyDeviceToHost,
+                                      e->stream));
+    } else if (keys) {
         CM_HIP_TRY(*e, hipMemcpyAsync(keys, k, (size_t)n * 8, hipMemcpyDeviceToHost, e->stream));
+    }
     if (order)
         CM_HIP_TRY(*e, hipMemcpyAsync(order, o, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
     const int nn = std::min(node_cap, hh.n_nodes);
@@ -927,6 +995,7 @@ extern "C" int32_t cm_test_tree(void* h, int32_t node_cap, uint64_t* keys, int32
                                       hipMemcpyDeviceToHost, e->stream));
     }
     CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
+    for (size_t q = 0; q < k32.size(); q++) keys[q] = k32[q];
     for (int q = 0; q < nn; q++) {
         if (info) {
             int32_t* r = info + 5 * (size_t)q;
