@@ -152,33 +152,44 @@ __device__ __forceinline__ uint32_t quantise(double x, double c, double h,
 }
 
 // Keys are stored as K: 32-bit words when 3 levels + 1 sentinel bit fit
-// (levels <= 10), else 64-bit.
+// (levels <= 10), else 64-bit.  One block per radix-sort tile (kSortTile
+// bodies, eight per thread): it also counts the tile's lowest key digit, the
+// histogram column of the sort's first pass.
 template <typename K>
-__global__ void __launch_bounds__(256)
-k_morton(int n, int levels, const TreeHeader* __restrict__ hdr,
+__global__ void __launch_bounds__(kSortThreads)
+k_morton(int n, int levels, int ntiles, const TreeHeader* __restrict__ hdr,
          const double* __restrict__ px, const double* __restrict__ py,
          const double* __restrict__ pz, const double* __restrict__ m,
          const uint8_t* __restrict__ act, K* __restrict__ keys,
          int32_t* __restrict__ vals, double4* __restrict__ pm4,
-         int32_t* __restrict__ gmark) {
-    int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= n) return;
-    gmark[i] = 0;   // target-group marks of the traversal, set after the build
-    uint64_t sentinel = (levels >= 21) ? ~0ull : ((2ull << (3 * levels)) - 1ull);
-    uint64_t k = sentinel;
-    double x = px[i], y = py[i], z = pz[i];
-    if (act[i]) {
-        double h = hdr->box[3];
-        uint32_t qx = quantise(x, hdr->box[0], h, levels);
-        uint32_t qy = quantise(y, hdr->box[1], h, levels);
-        uint32_t qz = quantise(z, hdr->box[2], h, levels);
-        k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
+         int32_t* __restrict__ gmark, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t h[256];
+    const int t = threadIdx.x;
+    h[t] = 0u;
+    __syncthreads();
+    const uint64_t sentinel = (levels >= 21) ? ~0ull : ((2ull << (3 * levels)) - 1ull);
+    const double bx = hdr->box[0], by = hdr->box[1], bz = hdr->box[2], bh = hdr->box[3];
+    for (int r = 0; r < kSortTile / kSortThreads; r++) {
+        const int i = blockIdx.x * kSortTile + r * kSortThreads + t;
+        if (i >= n) break;
+        gmark[i] = 0;   // target-group marks of the traversal, set after the build
+        uint64_t k = sentinel;
+        const double x = px[i], y = py[i], z = pz[i];
+        if (act[i]) {
+            const uint32_t qx = quantise(x, bx, bh, levels);
+            const uint32_t qy = quantise(y, by, bh, levels);
+            const uint32_t qz = quantise(z, bz, bh, levels);
+            k = (spread3(qx) << 2) | (spread3(qy) << 1) | spread3(qz);
+        }
+        keys[i] = (K)k;
+        vals[i] = i;
+        // position and mass packed per body, so that the gather after the sort
+        // reads one 32-byte record per body instead of four scattered words
+        pm4[i] = make_double4(x, y, z, m[i]);
+        atomicAdd(&h[(uint32_t)k & 255u], 1u);
     }
-    keys[i] = (K)k;
-    vals[i] = i;
-    // position and mass packed per body, so that the gather after the sort
-    // reads one 32-byte record per body instead of four scattered words
-    pm4[i] = make_double4(x, y, z, m[i]);
+    __syncthreads();
+    hist[(size_t)t * ntiles + blockIdx.x] = h[t];
 }
 
 // ---------------------------------------------------------- radix sort ----
@@ -846,16 +857,20 @@ k_tree_internal(int L, const TreeHeader* __restrict__ hdr, int4* __restrict__ in
 // first_pass .. first_pass + passes - 1.  On return *k/*v hold the sorted
 // pairs (the buffers are swapped after every pass); hist holds
 // 256 * (ceil(n / kSortTile) + 1) words.
+// first_counted: the caller has already written the first pass's
+// histogram columns.
 template <typename K>
 static void sort_pairs(hipStream_t st, int n, int first_pass, int passes, K** k,
-                       int32_t** v, K** kt, int32_t** vt, uint32_t* hist) {
+                       int32_t** v, K** kt, int32_t** vt, uint32_t* hist,
+                       bool first_counted = false) {
     if (n <= 0) return;
     int ntiles = (n + kSortTile - 1) / kSortTile;
     uint32_t* totals = hist + (size_t)256 * ntiles;
     for (int p = first_pass; p < first_pass + passes; p++) {
         int shift = 8 * p;
-        hipLaunchKernelGGL(k_sort_hist<K>, dim3(ntiles), dim3(kSortThreads), 0, st, n,
-                           shift, ntiles, *k, hist);
+        if (p > first_pass || !first_counted)
+            hipLaunchKernelGGL(k_sort_hist<K>, dim3(ntiles), dim3(kSortThreads), 0, st, n,
+                               shift, ntiles, *k, hist);
         hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(kSortThreads), 0, st, ntiles,
                            hist, totals);
         hipLaunchKernelGGL(k_sort_scatter<K>, dim3(ntiles), dim3(kSortThreads), 0, st,
@@ -877,15 +892,16 @@ static int build_keys_and_counts(Engine& e, int n, int levels, const int32_t** o
                                  const K** keys) {
     hipStream_t st = e.stream;
     const int gb = (n + 255) / 256;
+    const int stiles = (n + kSortTile - 1) / kSortTile;
     K* ka = reinterpret_cast<K*>(e.keys_a);
     K* kb = reinterpret_cast<K*>(e.keys_b);
-    hipLaunchKernelGGL(k_morton<K>, dim3(gb), dim3(256), 0, st, n, levels, e.hdr,
-                       e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, ka, e.vals_a,
-                       e.pm4, e.gmark);
+    hipLaunchKernelGGL(k_morton<K>, dim3(stiles), dim3(kSortThreads), 0, st, n, levels,
+                       stiles, e.hdr, e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, ka,
+                       e.vals_a, e.pm4, e.gmark, e.hist);
     if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
     const int passes = (3 * levels + 1 + 7) / 8;
     int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
-    sort_pairs<K>(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist);
+    sort_pairs<K>(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist, true);
     if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
     // source radii enter only the shell theorem's interior branch
     hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4,
