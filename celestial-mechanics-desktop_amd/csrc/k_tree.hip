@@ -7,13 +7,15 @@
 //   2. k_morton: levels x 3-bit Morton key per active body over that cube,
 //      x bit most significant in each group, so the top group is the C#
 //      octant number (OctreeNode.cs:185-192); inactive bodies get a sentinel
-//      that sorts after every real key.
+//      that sorts after every real key.  Keys are 32-bit words up to 10
+//      levels.  The kernel also counts the first sort pass's digits.
 //   3. Stable LSD radix sort of (key, body index), 8-bit digits: per-tile
 //      LDS digit histograms, one scan, and a scatter that ranks equal digits
 //      with 64-lane ballots in original order (stable).
-//   4. k_gather: sorted copies of position / mass (and radius, which only the
-//      shell theorem reads) for the tree and the traversal.
-//   5. k_tree_count / k_tree_scan / k_tree_emit: level-order octree from
+//   4. k_gather_count: sorted copies of position / mass (and radius, which
+//      only the shell theorem reads) for the tree and the traversal, and the
+//      per-tile node counts of step 5.
+//   5. k_gather_count / k_tree_scan / k_tree_emit: level-order octree from
 //      local comparisons of neighbouring keys (below).  A node with
 //      count > leaf_capacity below the deepest level has its non-empty
 //      children; children of a node are contiguous, every level is
@@ -376,22 +378,6 @@ k_sort_scatter(int n, int shift, int ntiles, const K* __restrict__ kin,
     }
 }
 
-// Sorted copies of x, y, z, m and, when the shell theorem needs source radii
-// (r != nullptr), of the radius.
-__global__ void __launch_bounds__(256)
-k_gather(int n, const int32_t* __restrict__ order, const double4* __restrict__ pm4,
-         const double* __restrict__ r, double* __restrict__ sorted, int cap) {
-    int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n) return;
-    int o = order[k];
-    double4 v = pm4[o];
-    sorted[k] = v.x;
-    sorted[(size_t)cap + k] = v.y;
-    sorted[2 * (size_t)cap + k] = v.z;
-    sorted[3 * (size_t)cap + k] = v.w;
-    if (r) sorted[4 * (size_t)cap + k] = r[o];
-}
-
 // ---------------------------------------------------------- tree build ----
 // The level-order octree follows from two local facts of the sorted keys:
 //   d[i] = leading 3-bit digits key[i] shares with key[i - 1] (0 for i = 0):
@@ -408,12 +394,12 @@ k_gather(int n, const int32_t* __restrict__ order, const double4* __restrict__ p
 // lam[i] = min(levels, 1 + max D[i - cap .. i]); the one of level lam[i] is a
 // leaf.  Nodes are numbered level by level in order of their first body (the
 // digit order of every parent's children), which is a per-level prefix count
-// of opened nodes over the bodies: k_tree_count counts them per tile,
+// of opened nodes over the bodies: k_gather_count counts them per tile,
 // k_tree_scan turns the counts into offsets, k_tree_emit writes the nodes and
 // the leaves' sums, k_tree_internal forms internal nodes bottom-up.
 constexpr int kTreeWaves = kTreeTile / kWave;
 // bodies staged past each tile: >= the largest leaf capacity (the D window of
-// k_tree_count), and large enough that deepest-level leaves, which may hold
+// k_gather_count), and large enough that deepest-level leaves, which may hold
 // more bodies than the leaf capacity (up to ~120 in the core of a 2^20-body
 // Plummer sphere with 30-bit keys), are usually summed from LDS
 constexpr int kTreeHalo = 128;
@@ -425,16 +411,32 @@ __device__ __forceinline__ int common_digits(uint64_t a, uint64_t b, int levels)
     return (3 * levels - 1 - p) / 3;
 }
 
+// One tree tile of the sorted order per block: the sorted copies of x, y, z,
+// m and, when the shell theorem needs source radii (r != nullptr), of the
+// radius for the tile's bodies (of all count bodies, inactive ones last),
+// then d, lam and the tile's node count per level over its active bodies.
 template <typename K>
 __global__ void __launch_bounds__(kTreeTile)
-k_tree_count(int levels, int cap, int tstride, const TreeHeader* __restrict__ hdr,
-             const K* __restrict__ keys, uchar2* __restrict__ dl,
-             uint32_t* __restrict__ tcount) {
+k_gather_count(int count, int levels, int cap, int tstride, const TreeHeader* __restrict__ hdr,
+               const K* __restrict__ keys, const int32_t* __restrict__ order,
+               const double4* __restrict__ pm4, const double* __restrict__ r,
+               double* __restrict__ sorted, int bcap, uchar2* __restrict__ dl,
+               uint32_t* __restrict__ tcount) {
     __shared__ signed char sD[kTreeTile + kTreeHalo];
     __shared__ uint32_t s_cnt[kMaxLevels + 1][kTreeWaves];
-    const int n = hdr->n_active;
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
     const int t0 = blockIdx.x * kTreeTile;
+    if (t0 + t < count) {
+        const int k = t0 + t;
+        const int o = order[k];
+        const double4 v = pm4[o];
+        sorted[k] = v.x;
+        sorted[(size_t)bcap + k] = v.y;
+        sorted[2 * (size_t)bcap + k] = v.z;
+        sorted[3 * (size_t)bcap + k] = v.w;
+        if (r) sorted[4 * (size_t)bcap + k] = r[o];
+    }
+    const int n = hdr->n_active;
     // D for j in [t0 - cap, t0 + kTreeTile)
     for (int q = t; q < kTreeTile + cap; q += kTreeTile) {
         int j = t0 - cap + q;
@@ -485,12 +487,7 @@ k_tree_scan(int levels, int cap, int ntiles, int tstride, int node_cap,
 #pragma unroll
         for (int l = 1; l <= kMaxLevels; l++) {
             uint32_t a = v[l];
-            if (l <= levels) {   // uniform
-                for (int o = 1; o < kWave; o <<= 1) {
-                    uint32_t y = __shfl_up(a, o, kWave);
-                    if (lane >= o) a += y;
-                }
-            }
+            if (l <= levels) a = wave_incl_scan(a);   // uniform: all lanes active
             x[l] = a;
             if (lane == kWave - 1) s_w[l][w] = a;
         }
@@ -891,7 +888,6 @@ template <typename K>
 static int build_keys_and_counts(Engine& e, int n, int levels, const int32_t** order,
                                  const K** keys) {
     hipStream_t st = e.stream;
-    const int gb = (n + 255) / 256;
     const int stiles = (n + kSortTile - 1) / kSortTile;
     K* ka = reinterpret_cast<K*>(e.keys_a);
     K* kb = reinterpret_cast<K*>(e.keys_b);
@@ -904,12 +900,11 @@ static int build_keys_and_counts(Engine& e, int n, int levels, const int32_t** o
     sort_pairs<K>(st, n, 0, passes, &ka, &va, &kb, &vb, e.hist, true);
     if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[2], st));
     // source radii enter only the shell theorem's interior branch
-    hipLaunchKernelGGL(k_gather, dim3(gb), dim3(256), 0, st, n, va, e.pm4,
-                       e.cfg.shell_theorem ? e.f[RAD] : nullptr, e.sorted, e.capacity);
     const int ntiles = std::max(1, (n + kTreeTile - 1) / kTreeTile);
-    hipLaunchKernelGGL(k_tree_count<K>, dim3(ntiles), dim3(kTreeTile), 0, st, levels,
-                       e.cfg.leaf_capacity, tree_tiles(e.capacity), e.hdr, ka, e.tree_dl,
-                       e.tree_tcount);
+    hipLaunchKernelGGL(k_gather_count<K>, dim3(ntiles), dim3(kTreeTile), 0, st, n, levels,
+                       e.cfg.leaf_capacity, tree_tiles(e.capacity), e.hdr, ka, va, e.pm4,
+                       e.cfg.shell_theorem ? e.f[RAD] : nullptr, e.sorted, e.capacity,
+                       e.tree_dl, e.tree_tcount);
     *order = va;
     *keys = ka;
     return CM_OK;
