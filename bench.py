@@ -45,6 +45,7 @@ FLOPS_PER_INTERACTION = 20
 FLOPS_PER_FAR_CELL = 76         # Taylor coefficients of one (group, cell) far-field term
 FAR_FIELD_RATIO = 0.25          # group far-field expansion ratio of the FP32 split path (swept: 0.15-0.3)
 LEAF_CAPACITY = 8               # bodies per leaf bucket (measured best at N = 2^20, theta 0.5-0.65)
+GROUP_SIZE = 64                 # bodies per target group (one wavefront's targets)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_traverse_plummer1M_traffic.md")
 
 
@@ -54,7 +55,7 @@ def log(msg):
 
 def make_backend(cmhip, n, precision, theta, params=None, far=0.0):
     be = cmhip.HipPhysicsBackend(n, precision=precision, theta=theta, morton_levels=10,
-                                 leaf_capacity=LEAF_CAPACITY, group_size=64,
+                                 leaf_capacity=LEAF_CAPACITY, group_size=GROUP_SIZE,
                                  far_field_ratio=far)
     if params is not None:
         be.mlp_set_params(params)
@@ -584,7 +585,7 @@ def only_line(args, cmhip):
             "dtype": "f64" if args.only == "fp64" else "f32-split (f64 state)",
             "data": "synthetic",
             "config": {"workload": "plummer_bh_verlet", "n_bodies": n, "theta": theta,
-                       "leaf_capacity": LEAF_CAPACITY, "seed": 42},
+                       "leaf_capacity": LEAF_CAPACITY, "group_size": GROUP_SIZE, "seed": 42},
             "kernel": kernel, "ms_kernel_avg": rec["ms_force_kernel_avg"],
             "interactions_per_body": rec["interactions_per_body"],
             "force_err_median": rec["force_err_median"]}
@@ -592,7 +593,7 @@ def only_line(args, cmhip):
 
 
 def main():
-    global LEAF_CAPACITY
+    global LEAF_CAPACITY, GROUP_SIZE
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -616,6 +617,8 @@ def main():
                     help="time only the FP32 theta = 0.5 line (profiling runs)")
     ap.add_argument("--leaf-capacity", type=int, default=LEAF_CAPACITY,
                     help="bodies per leaf bucket of the Barnes-Hut lines")
+    ap.add_argument("--group-size", type=int, default=GROUP_SIZE,
+                    help="bodies per target group of the Barnes-Hut paths (1..64)")
     ap.add_argument("--train-epochs", type=int, default=20,
                     help="epochs of the learned correction's training")
     ap.add_argument("--only", choices=("fast", "fp64", "learned"), default=None,
@@ -625,6 +628,7 @@ def main():
                     help="only run device copies of known size (PMC scale reference)")
     args = ap.parse_args()
     LEAF_CAPACITY = args.leaf_capacity
+    GROUP_SIZE = args.group_size
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -724,7 +728,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "plummer_bh_verlet", "n_bodies": n, "theta": args.theta,
                    "softening": eps, "dt": dt, "morton_bits": 30, "leaf_capacity": LEAF_CAPACITY,
-                   "plummer_mass": 1.0, "plummer_a_au": 1.0, "mass_fraction_cut": 0.999,
+                   "group_size": GROUP_SIZE, "plummer_mass": 1.0, "plummer_a_au": 1.0, "mass_fraction_cut": 0.999,
                    "far_field_ratio": far, "seed": 42},
         "force_err_median": fast["force_err_median"],
         "force_err_max": fast["force_err_max"],

@@ -19,6 +19,7 @@ constexpr int kStackCap = 1536;         // per-wave LDS traversal stack (ints)
 constexpr int kSortTile = 2048;         // keys per radix-sort tile
 constexpr int kSortThreads = 256;
 constexpr int kTreeTile = 1024;         // bodies per tree-build tile
+constexpr int kGroupTile = 1024;        // bodies per tile of the target-group ordering
 constexpr int kMaxDevices = 64;         // per-device tables of static device memory
 // largest body count of cm_energy_noalloc with a potential (its per-body rows
 // live in a static device array of this many doubles, 32 MiB per GPU)

@@ -51,10 +51,25 @@ __device__ __forceinline__ double vol_of(const double* lo, const double* hi, dou
     return ((hi[0] - lo[0]) + pad) * ((hi[1] - lo[1]) + pad) * ((hi[2] - lo[2]) + pad);
 }
 
-// A group is recorded at its first body (gmark[s] = e); k_group_order then
-// lists the groups in body (Morton) order, so that consecutive groups are
-// spatial neighbours.
-__device__ __forceinline__ void emit_group(int32_t* gmark, int s, int e) { gmark[s] = e; }
+// A group is recorded at its first body (gmark[s] = e) and counted in the
+// tile of kGroupTile bodies that holds it; k_group_emit then lists the groups
+// in body (Morton) order, so that consecutive groups are spatial neighbours.
+// A thread's groups are in body order, so it counts them per tile and adds
+// each tile's count once (the counter is cleared by k_morton).
+struct GroupCounter {
+    int32_t* gtile;
+    int tile = -1, cnt = 0;
+    __device__ void flush() {
+        if (cnt > 0) atomicAdd(&gtile[tile], cnt);
+        cnt = 0;
+    }
+    __device__ void emit(int32_t* gmark, int s, int e) {
+        gmark[s] = e;
+        const int t = s / kGroupTile;
+        if (t != tile) { flush(); tile = t; }
+        cnt++;
+    }
+};
 
 // box_split with the three boxes given directly: the bin [s, mid) as lo1/hi1,
 // the next child [mid, e) as lo2/hi2; the whole box is their union.  Node
@@ -76,20 +91,22 @@ __device__ bool box_split_nodes(const double* lo1, const double* hi1,
 
 __global__ void __launch_bounds__(256)
 k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
-         const double* __restrict__ nbox, int gsize, int32_t* __restrict__ gmark) {
+         const double* __restrict__ nbox, int gsize, int32_t* __restrict__ gmark,
+         int32_t* __restrict__ gtile) {
     // a tree that outgrew the node pool is incomplete: no groups, no
     // traversal (the evaluation reports CM_ERR_POOL_EXHAUSTED)
     if (hdr->overflow) return;
     const int nn = hdr->n_nodes;
+    GroupCounter gc{gtile};
     for (int k = blockIdx.x * 256 + threadIdx.x; k < nn; k += gridDim.x * 256) {
         int4 nd = info[k];
-        if (k == 0 && nd.w > 0 && nd.w <= gsize) emit_group(gmark, 0, nd.w);
+        if (k == 0 && nd.w > 0 && nd.w <= gsize) gc.emit(gmark, 0, nd.w);
         if (nd.w <= gsize) continue;
         if (nd.x < 0) {
             // a deepest-level leaf holding more than gsize bodies: consecutive
             // slices of gsize
             for (int q = nd.z; q < nd.z + nd.w; q += gsize)
-                emit_group(gmark, q, min(q + gsize, nd.z + nd.w));
+                gc.emit(gmark, q, min(q + gsize, nd.z + nd.w));
             continue;
         }
         int nch = nd.y & 255;
@@ -115,7 +132,7 @@ k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
                 open = true;
             }
             if (close && bs >= 0) {
-                emit_group(gmark, bs, be);
+                gc.emit(gmark, bs, be);
                 bs = -1;
             }
             if (open) {
@@ -123,29 +140,17 @@ k_groups(TreeHeader* __restrict__ hdr, const int4* __restrict__ info,
                 for (int q = 0; q < 3; q++) { blo[q] = cb[q]; bhi[q] = cb[3 + q]; }
             }
         }
-        if (bs >= 0) emit_group(gmark, bs, be);
+        if (bs >= 0) gc.emit(gmark, bs, be);
     }
+    gc.flush();
 }
 
-// Groups in body order: per tile of kGroupTile bodies count the marks, scan
-// the tile counts, then write each tile's groups at its offset (ballot ranks).
-constexpr int kGroupTile = 1024;
-
-__global__ void __launch_bounds__(kGroupTile)
-k_group_count(const TreeHeader* __restrict__ hdr, const int32_t* __restrict__ gmark,
-              int32_t* __restrict__ tcount) {
-    __shared__ int s_w[kGroupTile / kWave];
-    const int n = hdr->n_active;
-    const int i = blockIdx.x * kGroupTile + threadIdx.x;
-    const uint64_t m = __ballot(i < n && gmark[i] > 0);
-    if ((threadIdx.x & (kWave - 1)) == 0) s_w[threadIdx.x / kWave] = __popcll(m);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int w = 0; w < kGroupTile / kWave; w++) s += s_w[w];
-        tcount[blockIdx.x] = s;
-    }
-}
+// Groups in body order: k_groups counts the groups of every tile of
+// kGroupTile bodies; up to kGroupPrefixTiles tiles, each block of k_group_emit
+// sums the counts of the tiles before its own, else one scan kernel turns the
+// counts into offsets first; each tile's groups are then written at its
+// offset (ballot ranks).
+constexpr int kGroupPrefixTiles = 4096;
 
 __global__ void __launch_bounds__(1024)
 k_group_scan(int ntiles, TreeHeader* __restrict__ hdr, int32_t* __restrict__ tcount) {
@@ -175,25 +180,52 @@ k_group_scan(int ntiles, TreeHeader* __restrict__ hdr, int32_t* __restrict__ tco
         if (t == 0) s_carry = carry + tot;
         __syncthreads();
     }
-    if (t == 0) {
-        hdr->n_groups = s_carry;
-        for (int k = 0; k < kXcds; k++) hdr->work_next[k] = 0u;
-    }
+    if (t == 0) hdr->n_groups = s_carry;
 }
 
+// scanned: tcount holds each tile's offset and hdr->n_groups the total
+// (k_group_scan ran); else tcount holds each tile's count.
 __global__ void __launch_bounds__(kGroupTile)
-k_group_emit(const TreeHeader* __restrict__ hdr, const int32_t* __restrict__ gmark,
-             const int32_t* __restrict__ toff, int2* __restrict__ groups) {
-    __shared__ int s_w[kGroupTile / kWave];
+k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
+             const int32_t* __restrict__ gmark, const int32_t* __restrict__ tcount,
+             int2* __restrict__ groups) {
+    constexpr int kW = kGroupTile / kWave;
+    __shared__ int s_w[kW];   // marks per wave of this tile
+    __shared__ int s_b[kW];   // per wave: counts of the tiles before this one
+    __shared__ int s_a[kW];   // per wave: counts of all tiles
     const int n = hdr->n_active;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int i = blockIdx.x * kGroupTile + threadIdx.x;
     const int e = i < n ? gmark[i] : 0;
+    int before = 0, all = 0;
+    if (!scanned) {
+        for (int q = threadIdx.x; q < ntiles; q += kGroupTile) {
+            const int c = tcount[q];
+            before += q < (int)blockIdx.x ? c : 0;
+            all += c;
+        }
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            before += __shfl_xor(before, o, kWave);
+            all += __shfl_xor(all, o, kWave);
+        }
+    }
     const uint64_t m = __ballot(e > 0);
-    if (lane == 0) s_w[w] = __popcll(m);
+    if (lane == 0) {
+        s_w[w] = __popcll(m);
+        s_b[w] = before;
+        s_a[w] = all;
+    }
     __syncthreads();
+    int base = 0;
+    if (scanned) {
+        base = tcount[blockIdx.x];
+    } else {
+        int tot = 0;
+        for (int k = 0; k < kW; k++) { base += s_b[k]; tot += s_a[k]; }
+        if (blockIdx.x == 0 && threadIdx.x == 0) hdr->n_groups = tot;
+    }
     if (e > 0) {
-        int off = toff[blockIdx.x];
+        int off = base;
         for (int k = 0; k < w; k++) off += s_w[k];
         off += __popcll(m & ((1ull << lane) - 1ull));
         groups[off] = make_int2(i, e);
@@ -936,13 +968,15 @@ int form_groups(Engine& e) {
     const int n = e.count;
     const int gb = std::min(1024, std::max(1, (n + 255) / 256));
     const int tiles = std::max(1, (n + kGroupTile - 1) / kGroupTile);
+    // group tile counters are cleared by k_morton
     hipLaunchKernelGGL(k_groups, dim3(gb), dim3(256), 0, e.stream, e.hdr, e.nd_info,
-                       e.nd_box, e.cfg.group_size, e.gmark);
-    hipLaunchKernelGGL(k_group_count, dim3(tiles), dim3(kGroupTile), 0, e.stream, e.hdr,
-                       e.gmark, e.gtile);
-    hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e.stream, tiles, e.hdr, e.gtile);
-    hipLaunchKernelGGL(k_group_emit, dim3(tiles), dim3(kGroupTile), 0, e.stream, e.hdr,
-                       e.gmark, e.gtile, e.groups);
+                       e.nd_box, e.cfg.group_size, e.gmark, e.gtile);
+    const int scanned = tiles > kGroupPrefixTiles ? 1 : 0;
+    if (scanned)
+        hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e.stream, tiles, e.hdr,
+                           e.gtile);
+    hipLaunchKernelGGL(k_group_emit, dim3(tiles), dim3(kGroupTile), 0, e.stream, tiles,
+                       scanned, e.hdr, e.gmark, e.gtile, e.groups);
     CM_HIP_TRY(e, hipGetLastError());
     return CM_OK;
 }

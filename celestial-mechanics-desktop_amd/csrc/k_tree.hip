@@ -164,10 +164,16 @@ k_morton(int n, int levels, int ntiles, const TreeHeader* __restrict__ hdr,
          const double* __restrict__ pz, const double* __restrict__ m,
          const uint8_t* __restrict__ act, K* __restrict__ keys,
          int32_t* __restrict__ vals, double4* __restrict__ pm4,
-         int32_t* __restrict__ gmark, uint32_t* __restrict__ hist) {
+         int32_t* __restrict__ gmark, int32_t* __restrict__ gtile,
+         uint32_t* __restrict__ hist) {
     __shared__ uint32_t h[256];
     const int t = threadIdx.x;
     h[t] = 0u;
+    // the target-group counters of this block's group tiles (counted by k_groups)
+    if (t < kSortTile / kGroupTile) {
+        const int g = blockIdx.x * (kSortTile / kGroupTile) + t;
+        if (g * kGroupTile < n) gtile[g] = 0;
+    }
     __syncthreads();
     const uint64_t sentinel = (levels >= 21) ? ~0ull : ((2ull << (3 * levels)) - 1ull);
     const double bx = hdr->box[0], by = hdr->box[1], bz = hdr->box[2], bh = hdr->box[3];
@@ -893,7 +899,7 @@ static int build_keys_and_counts(Engine& e, int n, int levels, const int32_t** o
     K* kb = reinterpret_cast<K*>(e.keys_b);
     hipLaunchKernelGGL(k_morton<K>, dim3(stiles), dim3(kSortThreads), 0, st, n, levels,
                        stiles, e.hdr, e.f[PX], e.f[PY], e.f[PZ], e.f[MASS], e.act, ka,
-                       e.vals_a, e.pm4, e.gmark, e.hist);
+                       e.vals_a, e.pm4, e.gmark, e.gtile, e.hist);
     if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[1], st));
     const int passes = (3 * levels + 1 + 7) / 8;
     int32_t* va = e.vals_a; int32_t* vb = e.vals_b;
