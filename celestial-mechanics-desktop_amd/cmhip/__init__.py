@@ -130,7 +130,8 @@ class ForceStats(C.Structure):
                 ("ms_force", C.c_float), ("ms_total", C.c_float),
                 ("acc_interactions", C.c_int64), ("acc_ms_force", C.c_double),
                 ("acc_ms_total", C.c_double), ("acc_evaluations", C.c_int32),
-                ("far_cells", C.c_int64), ("acc_far_cells", C.c_int64)]
+                ("far_cells", C.c_int64), ("acc_far_cells", C.c_int64),
+                ("acc_timed_evaluations", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

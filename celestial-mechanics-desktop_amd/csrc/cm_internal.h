@@ -64,6 +64,9 @@ struct TreeHeader {
 constexpr int kXcds = 8;
 
 constexpr int kWindow = 1024;     // force evaluations timed per host sync
+// every kTimingStride-th force evaluation of the engine is timed with the
+// three window events (each event record holds the stream for a few us)
+constexpr int kTimingStride = 4;
 
 struct Engine {
     int device = 0;
@@ -137,17 +140,26 @@ struct Engine {
     bool detail_timing() const { return timing && stats_eval; }
     hipEvent_t ev[6] = {};
     cm_force_stats stats{};
-    // per-evaluation (start, force-kernel start, end) events since the last
-    // host sync, folded into the accumulators by finalize_stats
+    // per-evaluation (start, force-kernel start, end) events of the timed
+    // evaluations since the last host sync, folded into the accumulators by
+    // finalize_stats; evals_since counts every evaluation
     hipEvent_t* wev = nullptr;        // 3 x kWindow
     int wn = 0;
+    int evals_since = 0;
+    long long eval_seq = 0;           // force evaluations since cm_create
     double acc_ms_force = 0.0, acc_ms_total = 0.0;
-    int acc_evals = 0;
+    int acc_evals = 0, acc_timed = 0;
 
-    // records the three window events of one force evaluation
+    // marks phase `which` (0 start, 1 force kernel, 2 end) of one force
+    // evaluation: the events are recorded for every kTimingStride-th one
     void mark(int which) {
-        if (wn < kWindow) (void)hipEventRecord(wev[3 * wn + which], stream);
-        if (which == 2 && wn < kWindow) wn++;
+        const bool timed = eval_seq % kTimingStride == 0 && wn < kWindow;
+        if (timed) (void)hipEventRecord(wev[3 * wn + which], stream);
+        if (which == 2) {
+            if (timed) wn++;
+            evals_since++;
+            eval_seq++;
+        }
     }
 };
 

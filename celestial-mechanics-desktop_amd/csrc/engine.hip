@@ -59,15 +59,18 @@ int finalize_stats(Engine& e) {
             hipEventElapsedTime(&b, e.wev[3 * w], e.wev[3 * w + 2]) == hipSuccess) {
             e.acc_ms_force += a;
             e.acc_ms_total += b;
-            e.acc_evals++;
+            e.acc_timed++;
         }
     }
     e.wn = 0;
+    e.acc_evals += e.evals_since;
+    e.evals_since = 0;
     s.acc_interactions = (int64_t)e.hdr_host->acc_int;
     s.acc_far_cells = (int64_t)e.hdr_host->acc_far;
     s.acc_ms_force = e.acc_ms_force;
     s.acc_ms_total = e.acc_ms_total;
     s.acc_evaluations = e.acc_evals;
+    s.acc_timed_evaluations = e.acc_timed;
     e.stats = s;
     return CM_OK;
 }
@@ -674,11 +677,13 @@ int32_t cm_reset_force_stats(void* h) {
     e->acc_ms_force = 0.0;
     e->acc_ms_total = 0.0;
     e->acc_evals = 0;
+    e->acc_timed = 0;
     e->stats.acc_interactions = 0;
     e->stats.acc_far_cells = 0;
     e->stats.acc_ms_force = 0.0;
     e->stats.acc_ms_total = 0.0;
     e->stats.acc_evaluations = 0;
+    e->stats.acc_timed_evaluations = 0;
     return CM_OK;
 }
 

@@ -107,14 +107,15 @@ typedef struct {
     float ms_sort;              /* radix sort only */
     float ms_force;             /* traversal / direct kernel */
     float ms_total;             /* whole force evaluation */
-    /* accumulated since the last cm_reset_force_stats (up to 1024 evaluations
-     * per accumulation window are timed): */
+    /* accumulated since the last cm_reset_force_stats: */
     int64_t acc_interactions;   /* cell + body terms */
-    double acc_ms_force;        /* summed force-kernel time, HIP events */
-    double acc_ms_total;        /* summed force-evaluation time */
-    int32_t acc_evaluations;
+    double acc_ms_force;        /* summed force-kernel time of the timed evaluations, HIP events */
+    double acc_ms_total;        /* summed force-evaluation time of the timed evaluations */
+    int32_t acc_evaluations;    /* every force evaluation */
     int64_t far_cells;          /* (group, cell) far-field expansion terms */
     int64_t acc_far_cells;      /* the same, accumulated */
+    int32_t acc_timed_evaluations; /* the timed ones: every 4th evaluation of the engine
+                                    * (at most 1024 between two host synchronisations) */
 } cm_force_stats;
 
 /* EnergyCalculator quantities (Validation/EnergyCalculator.cs:16-81) with
