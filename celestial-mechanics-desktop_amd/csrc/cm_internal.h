@@ -64,8 +64,9 @@ struct TreeHeader {
 constexpr int kXcds = 8;
 
 constexpr int kWindow = 1024;     // force evaluations timed per host sync
-// every kTimingStride-th force evaluation of the engine is timed with the
-// three window events (each event record holds the stream for a few us)
+// the first force evaluation of each API call and every kTimingStride-th
+// one after it are timed with the three window events (each event record
+// holds the stream for a few us)
 constexpr int kTimingStride = 4;
 
 struct Engine {
@@ -146,7 +147,7 @@ struct Engine {
     hipEvent_t* wev = nullptr;        // 3 x kWindow
     int wn = 0;
     int evals_since = 0;
-    long long eval_seq = 0;           // force evaluations since cm_create
+    long long eval_seq = 0;           // force evaluations since the call began
     double acc_ms_force = 0.0, acc_ms_total = 0.0;
     int acc_evals = 0, acc_timed = 0;
 

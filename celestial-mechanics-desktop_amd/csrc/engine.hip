@@ -83,6 +83,7 @@ __global__ void k_reset_acc(TreeHeader* hdr) {
 __global__ void k_begin_run(TreeHeader* hdr) { hdr->overflow_run = 0; }
 
 int begin_run(Engine& e) {
+    e.eval_seq = 0;   // the first force evaluation of every call is timed
     if (e.cfg.kind != CM_BACKEND_BARNES_HUT || e.count == 0) return CM_OK;
     hipLaunchKernelGGL(k_begin_run, dim3(1), dim3(1), 0, e.stream, e.hdr);
     CM_HIP_TRY(e, hipGetLastError());

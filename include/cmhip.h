@@ -114,8 +114,9 @@ typedef struct {
     int32_t acc_evaluations;    /* every force evaluation */
     int64_t far_cells;          /* (group, cell) far-field expansion terms */
     int64_t acc_far_cells;      /* the same, accumulated */
-    int32_t acc_timed_evaluations; /* the timed ones: every 4th evaluation of the engine
-                                    * (at most 1024 between two host synchronisations) */
+    int32_t acc_timed_evaluations; /* the timed ones: the first evaluation of each call and
+                                    * every 4th after it (at most 1024 between two host
+                                    * synchronisations) */
 } cm_force_stats;
 
 /* EnergyCalculator quantities (Validation/EnergyCalculator.cs:16-81) with
