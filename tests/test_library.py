@@ -54,3 +54,44 @@ def test_plummer_generator_is_deterministic_and_truncates():
     r = np.sqrt(a.pos_x ** 2 + a.pos_y ** 2 + a.pos_z ** 2)
     # 0.999 mass fraction: r <= a / sqrt(0.999^(-2/3) - 1) = 38.7 a (before COM shift)
     assert r.max() < 39.0
+
+
+def _handle_functions():
+    """Entry points whose first parameter is an engine or ensemble handle."""
+    hdr = open(os.path.join(ROOT, "include", "cmhip.h")).read()
+    decl = re.findall(r"^(int32_t|const char\*)\s+(cm_\w+)\s*\(\s*void\*\s*(\w+)", hdr, re.M)
+    return [(ret, name) for ret, name, _ in decl]
+
+
+def test_null_handles_are_rejected_before_any_device_call():
+    """Every call on a NULL engine or ensemble handle returns
+    CM_ERR_NOT_INITIALIZED (the reference's ErrorCode::NotInitialized,
+    engine/include/celestial/core/error.hpp:9-18) and touches no device: the
+    check comes first, so this runs without a GPU."""
+    import ctypes as C
+    L = C.CDLL(os.path.join(PKG, "libcmhip.so"))
+    funcs = _handle_functions()
+    assert len(funcs) > 30
+    for ret, name in funcs:
+        fn = getattr(L, name)
+        if ret == "const char*":
+            fn.restype = C.c_char_p
+            fn.argtypes = [C.c_void_p]
+            assert fn(None) is not None, name      # a message, never a NULL pointer
+            continue
+        fn.restype = C.c_int32
+        # the handle check precedes every other argument's use; the rest are
+        # zero / NULL (pointer-sized integer registers)
+        assert fn(None, *([None] * 8)) == cmhip.CM_ERR_NOT_INITIALIZED, name
+
+
+def test_create_rejects_bad_arguments_without_a_device():
+    import ctypes as C
+    L = C.CDLL(os.path.join(PKG, "libcmhip.so"))
+    L.cm_create.argtypes = [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]
+    L.cm_create.restype = C.c_int32
+    h = C.c_void_p()
+    assert L.cm_create(0, 0, C.byref(h)) == cmhip.CM_ERR_INVALID_ARGUMENT
+    assert L.cm_create(-5, 0, C.byref(h)) == cmhip.CM_ERR_INVALID_ARGUMENT
+    assert L.cm_create(16, 0, None) == cmhip.CM_ERR_INVALID_ARGUMENT
+    assert h.value is None
