@@ -2,9 +2,9 @@
  * cm_oracle.h -- CPU oracle for the celestial-mechanics Barnes-Hut hot path.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
- * calls this code.  It is used by tests/, by __graft_entry__.smoke() as the
- * checker, and by bench.py's cpu_baseline leg (timed as the CPU restatement of
- * the reference's FP64 path).
+ * calls this code.  It is used by tests/, by __graft_entry__.smoke() to check
+ * its result, and by bench.py's cpu_baseline leg (timed as the CPU restatement
+ * of the reference's FP64 path).
  *
  * Every function restates either
  *   (a) the reference's C# FP64 physics path (file:line cited per function), or

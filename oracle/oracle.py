@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (liboracle_cm.so).
 
-TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the
-checker and by bench.py's cpu_baseline leg.  The product (libcmhip.so and the
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() to
+check its result and by bench.py's cpu_baseline leg.  The product (libcmhip.so and the
 cmhip package) never imports it.
 
 Each wrapper returns numpy arrays; see cm_oracle.c for the reference citations
