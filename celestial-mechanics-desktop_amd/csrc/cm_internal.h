@@ -201,6 +201,9 @@ const float* mlp_params(const Engine& e);       // FP32 master (biases)
 void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
                       uint64_t** k, int32_t** v, uint64_t** kt, int32_t** vt,
                       uint32_t* hist);
+void radix_sort_pairs32(hipStream_t st, int n, int first_pass, int passes,
+                        uint32_t** k, int32_t** v, uint32_t** kt, int32_t** vt,
+                        uint32_t* hist);
 
 int set_error(Engine& e, int code, const std::string& msg);
 int finalize_stats(Engine& e);   // syncs the stream, fills e.stats

@@ -6,11 +6,15 @@
 //     SpatialHashBroadPhase (SpatialHashBroadPhase.cs:25-33); a body's cell key
 //     packs floor(x / cell) per axis into 21-bit fields (wrapping: aliased
 //     cells only add candidates, never lose one);
-//   * active bodies sorted by (cell key, index) with the radix sort of the
-//     tree build; every active, collidable body scans the 27 neighbouring
-//     cells (9 key ranges) and runs TryAddEvent's narrow phase against each
-//     candidate of larger index, so the event set is exactly the i < j pairs
-//     with d^2 < (r_i + r_j)^2 -- what the spatial hash yields;
+//   * each cell key is hashed to one of 2^B buckets (B = log2(capacity) + 2,
+//     so most buckets hold one cell); active bodies are sorted by (bucket,
+//     index) with the 32-bit radix sort of the tree build and each bucket's
+//     first position is recorded; every active, collidable body looks up the
+//     buckets of its 27 neighbouring cells (each distinct bucket once) and
+//     runs TryAddEvent's narrow phase against each candidate of larger index.
+//     Buckets shared by several cells only add candidates, which the narrow
+//     phase rejects, so the event set is exactly the i < j pairs with
+//     d^2 < (r_i + r_j)^2 -- what the spatial hash yields;
 //   * events are ordered like the reference's: (i, j) ascending, then a
 //     stable sort by overlap depth, deepest first (two radix sorts, or one
 //     bitonic sort in LDS for up to 2048 events).
@@ -71,6 +75,11 @@ struct Coll {
     uint8_t* caps_act = nullptr;   // merge caps: active flags as the decision pass sees them
     int32_t* caps_count = nullptr; // merge caps: merges per body this call
     uint8_t* caps_allow = nullptr; // merge caps: per event
+    int32_t* bstart = nullptr;     // per cell bucket: first sorted position (stale
+                                   // entries fail the key check of the lookup)
+    double4* sp = nullptr;         // x, y, z, radius of the active bodies in bucket order
+    uint8_t* scol = nullptr;       // their collidable flags
+    int bucket_bits = 0;
     int n_events = 0;              // of the last detection
 };
 
@@ -81,7 +90,8 @@ void collide_free(Engine& e) {
     if (!c) return;
     void* ptrs[] = {c->ka, c->kb, c->va, c->vb, c->hist, c->raw, c->ev, c->eka,
                     c->ekb, c->eva, c->evb, c->first, c->done, c->hdr, c->scan,
-                    c->new_index, c->caps_act, c->caps_count, c->caps_allow};
+                    c->new_index, c->caps_act, c->caps_count, c->caps_allow, c->bstart,
+                    c->sp, c->scol};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     delete c;
     e.coll = nullptr;
@@ -117,6 +127,13 @@ static int ensure_coll(Engine& e) {
     alloc((void**)&c->hdr, sizeof(CollHeader));
     alloc((void**)&c->scan, ((n + 255) / 256 + 1) * 4);
     alloc((void**)&c->new_index, n * 4);
+    int bits = 10;
+    while (bits < 26 && ((size_t)1 << bits) < 4 * n) bits++;
+    c->bucket_bits = bits;
+    alloc((void**)&c->bstart, ((size_t)1 << bits) * 4);
+    alloc((void**)&c->sp, n * sizeof(double4));
+    alloc((void**)&c->scol, n);
+    if (ok && hipMemset(c->bstart, 0xff, ((size_t)1 << bits) * 4) != hipSuccess) ok = false;
     if (!ok) {
         collide_free(e);
         return set_error(e, CM_ERR_OUT_OF_MEMORY, "collision scratch allocation failed");
@@ -191,29 +208,53 @@ k_max_radius(int nparts, const double* __restrict__ part_r, const int32_t* __res
     }
 }
 
+// Bucket of a cell key: the top `bits` bits of the SplitMix64 finaliser of
+// the key, so that neighbouring cells land in unrelated buckets.
+__device__ __forceinline__ uint32_t cell_bucket(uint64_t key, int bits) {
+    uint64_t z = key + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (uint32_t)(z >> (64 - bits));
+}
+
+// Bucket of each active body's cell (inactive bodies: 2^bits, after every
+// bucket) and its index, the pairs the radix sort orders.
 __global__ void __launch_bounds__(256)
-k_cell_keys(int n, const double* __restrict__ px, const double* __restrict__ py,
+k_cell_keys(int n, int bits, const double* __restrict__ px, const double* __restrict__ py,
             const double* __restrict__ pz, const uint8_t* __restrict__ act,
-            const CollHeader* __restrict__ h, uint64_t* __restrict__ keys,
+            const CollHeader* __restrict__ h, uint32_t* __restrict__ keys,
             int32_t* __restrict__ vals) {
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
-    uint64_t k = ~0ull;
+    uint32_t k = 1u << bits;
     if (act[i]) {
         double inv = h->inv_cell;
-        k = cell_key(cell_coord(px[i], inv), cell_coord(py[i], inv), cell_coord(pz[i], inv));
+        k = cell_bucket(cell_key(cell_coord(px[i], inv), cell_coord(py[i], inv),
+                                 cell_coord(pz[i], inv)), bits);
     }
     keys[i] = k;
     vals[i] = i;
 }
 
-__device__ __forceinline__ int lower_key(const uint64_t* __restrict__ k, int n, uint64_t x) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (k[mid] < x) lo = mid + 1; else hi = mid;
-    }
-    return lo;
+// First sorted position of every bucket present among the active bodies, and
+// the bodies' positions, radii and collidable flags in sorted order (so that
+// the candidates of one bucket are read from consecutive addresses).
+__global__ void __launch_bounds__(256)
+k_bucket_starts(const CollHeader* __restrict__ h, const uint32_t* __restrict__ keys,
+                const int32_t* __restrict__ vals, const double* __restrict__ px,
+                const double* __restrict__ py, const double* __restrict__ pz,
+                const double* __restrict__ r, const uint8_t* __restrict__ col,
+                int32_t* __restrict__ bstart, double4* __restrict__ sp,
+                uint8_t* __restrict__ scol) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const int na = h->n_active;
+    if (p >= na) return;
+    const uint32_t b = keys[p];
+    if (p == 0 || keys[p - 1] != b) bstart[b] = p;
+    const int i = vals[p];
+    sp[p] = make_double4(px[i], py[i], pz[i], r[i]);
+    scol[p] = col[i];
 }
 
 // CollisionDetector.TryAddEvent (CollisionDetector.cs:70-135) for i < j.
@@ -252,39 +293,47 @@ __device__ __forceinline__ void try_event(int i, int j, const double* px, const 
     }
 }
 
+// One active, collidable body per thread (sorted position p).  The distance
+// test runs on the sorted copies; a pair that overlaps goes through
+// try_event on the body arrays (the same values, so the same event).
 __global__ void __launch_bounds__(256)
-k_detect(int n, const uint64_t* __restrict__ keys, const int32_t* __restrict__ vals,
+k_detect(int n, int bits, const uint32_t* __restrict__ keys, const int32_t* __restrict__ vals,
+         const int32_t* __restrict__ bstart, const double4* __restrict__ sp,
+         const uint8_t* __restrict__ scol,
          const double* __restrict__ px, const double* __restrict__ py,
          const double* __restrict__ pz, const double* __restrict__ vx,
          const double* __restrict__ vy, const double* __restrict__ vz,
          const double* __restrict__ m, const double* __restrict__ r,
-         const uint8_t* __restrict__ col, CollHeader* __restrict__ h,
-         CollEvent* __restrict__ out, int cap) {
+         CollHeader* __restrict__ h, CollEvent* __restrict__ out, int cap) {
     int p = blockIdx.x * 256 + threadIdx.x;
     const int na = h->n_active;
     if (p >= na) return;
-    int i = vals[p];
-    if (!col[i]) return;
+    if (!scol[p]) return;
+    const int i = vals[p];
+    const double4 me = sp[p];
     double inv = h->inv_cell;
-    int64_t cx = cell_coord(px[i], inv), cy = cell_coord(py[i], inv), cz = cell_coord(pz[i], inv);
+    int64_t cx = cell_coord(me.x, inv), cy = cell_coord(me.y, inv), cz = cell_coord(me.z, inv);
+    // the 27 neighbouring cells' buckets, each distinct bucket scanned once
+    uint32_t seen[27];
+    int ns = 0;
     for (int ddx = -1; ddx <= 1; ddx++) {
         for (int ddy = -1; ddy <= 1; ddy++) {
-            uint64_t k0 = cell_key(cx + ddx, cy + ddy, cz - 1);
-            uint64_t k2 = cell_key(cx + ddx, cy + ddy, cz + 1);
-            int nr = 1;
-            uint64_t rl[3], rh[3];
-            if (k2 == k0 + 2ull) { rl[0] = k0; rh[0] = k2; }
-            else {
-                nr = 3;
-                for (int dz = -1; dz <= 1; dz++) {
-                    uint64_t kk = cell_key(cx + ddx, cy + ddy, cz + dz);
-                    rl[dz + 1] = kk; rh[dz + 1] = kk;
-                }
-            }
-            for (int q = 0; q < nr; q++) {
-                for (int s = lower_key(keys, na, rl[q]); s < na && keys[s] <= rh[q]; s++) {
-                    int j = vals[s];
-                    if (j <= i || !col[j]) continue;
+            for (int ddz = -1; ddz <= 1; ddz++) {
+                const uint32_t bk = cell_bucket(cell_key(cx + ddx, cy + ddy, cz + ddz), bits);
+                bool dup = false;
+                for (int q = 0; q < ns; q++) dup = dup || seen[q] == bk;
+                if (dup) continue;
+                seen[ns++] = bk;
+                int s = bstart[bk];
+                if (s < 0 || s >= na || keys[s] != bk) continue;   // bucket empty
+                for (; s < na && keys[s] == bk; s++) {
+                    const int j = vals[s];
+                    if (j <= i || !scol[s]) continue;
+                    const double4 o = sp[s];
+                    const double dx = me.x - o.x, dy = me.y - o.y, dz = me.z - o.z;
+                    const double dist2 = dx * dx + dy * dy + dz * dz;
+                    const double sumr = me.w + o.w;
+                    if (dist2 >= sumr * sumr) continue;
                     try_event(i, j, px, py, pz, vx, vy, vz, m, r, h, out, cap);
                 }
             }
@@ -395,14 +444,18 @@ int collide_detect(Engine& e, cm_collision_event* out, int32_t cap, int32_t* ne_
                            c->hdr);
     }
     int gb = (n + 255) / 256;
-    hipLaunchKernelGGL(k_cell_keys, dim3(gb), dim3(256), 0, st, n, e.f[PX], e.f[PY],
-                       e.f[PZ], e.act, c->hdr, c->ka, c->va);
-    uint64_t *k = c->ka, *kt = c->kb;
+    const int bits = c->bucket_bits;
+    uint32_t* k = reinterpret_cast<uint32_t*>(c->ka);
+    uint32_t* kt = reinterpret_cast<uint32_t*>(c->kb);
     int32_t *v = c->va, *vt = c->vb;
-    radix_sort_pairs(st, n, 0, 8, &k, &v, &kt, &vt, c->hist);
-    hipLaunchKernelGGL(k_detect, dim3(gb), dim3(256), 0, st, n, k, v, e.f[PX], e.f[PY],
-                       e.f[PZ], e.f[VX], e.f[VY], e.f[VZ], e.f[MASS], e.f[RAD], e.col,
-                       c->hdr, c->raw, c->cap_events);
+    hipLaunchKernelGGL(k_cell_keys, dim3(gb), dim3(256), 0, st, n, bits, e.f[PX], e.f[PY],
+                       e.f[PZ], e.act, c->hdr, k, v);
+    radix_sort_pairs32(st, n, 0, (bits + 1 + 7) / 8, &k, &v, &kt, &vt, c->hist);
+    hipLaunchKernelGGL(k_bucket_starts, dim3(gb), dim3(256), 0, st, c->hdr, k, v, e.f[PX],
+                       e.f[PY], e.f[PZ], e.f[RAD], e.col, c->bstart, c->sp, c->scol);
+    hipLaunchKernelGGL(k_detect, dim3(gb), dim3(256), 0, st, n, bits, k, v, c->bstart, c->sp,
+                       c->scol, e.f[PX], e.f[PY], e.f[PZ], e.f[VX], e.f[VY], e.f[VZ],
+                       e.f[MASS], e.f[RAD], c->hdr, c->raw, c->cap_events);
     CM_HIP_TRY(e, hipGetLastError());
     CollHeader hh;
     CM_HIP_TRY(e, hipMemcpyAsync(&hh, c->hdr, sizeof(hh), hipMemcpyDeviceToHost, st));

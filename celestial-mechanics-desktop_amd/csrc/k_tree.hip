@@ -889,6 +889,12 @@ void radix_sort_pairs(hipStream_t st, int n, int first_pass, int passes,
     sort_pairs<uint64_t>(st, n, first_pass, passes, k, v, kt, vt, hist);
 }
 
+void radix_sort_pairs32(hipStream_t st, int n, int first_pass, int passes,
+                        uint32_t** k, int32_t** v, uint32_t** kt, int32_t** vt,
+                        uint32_t* hist) {
+    sort_pairs<uint32_t>(st, n, first_pass, passes, k, v, kt, vt, hist);
+}
+
 // Keys, sort and node counts of the build for K-wide keys.
 template <typename K>
 static int build_keys_and_counts(Engine& e, int n, int levels, const int32_t** order,
