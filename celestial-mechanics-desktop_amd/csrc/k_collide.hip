@@ -6,8 +6,9 @@
 //     SpatialHashBroadPhase (SpatialHashBroadPhase.cs:25-33); a body's cell key
 //     packs floor(x / cell) per axis into 21-bit fields (wrapping: aliased
 //     cells only add candidates, never lose one);
-//   * each cell key is hashed to one of 2^B buckets (B = log2(capacity) + 2,
-//     so most buckets hold one cell); active bodies are sorted by (bucket,
+//   * each cell goes to one of 2^B buckets (B = log2(capacity) + 2, so most
+//     buckets hold one cell; the cells of a 4x4x4 block get consecutive
+//     buckets, the blocks hashed); active bodies are sorted by (bucket,
 //     index) with the 32-bit radix sort of the tree build and each bucket's
 //     first position is recorded; every active, collidable body looks up the
 //     buckets of its 27 neighbouring cells (each distinct bucket once) and
@@ -208,14 +209,19 @@ k_max_radius(int nparts, const double* __restrict__ part_r, const int32_t* __res
     }
 }
 
-// Bucket of a cell key: the top `bits` bits of the SplitMix64 finaliser of
-// the key, so that neighbouring cells land in unrelated buckets.
-__device__ __forceinline__ uint32_t cell_bucket(uint64_t key, int bits) {
-    uint64_t z = key + 0x9e3779b97f4a7c15ull;
+// Bucket of cell (cx, cy, cz): cells are grouped in blocks of 4 x 4 x 4; the
+// block's packed key is hashed (top bits - 6 bits of the SplitMix64
+// finaliser) and the cell's place in its block fills the low 6 bits, so the
+// cells of one block have consecutive buckets and the bodies of a block are
+// contiguous after the sort (neighbouring threads then share most of their
+// neighbour cells).
+__device__ __forceinline__ uint32_t cell_bucket(int64_t cx, int64_t cy, int64_t cz, int bits) {
+    uint64_t z = cell_key(cx >> 2, cy >> 2, cz >> 2) + 0x9e3779b97f4a7c15ull;
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     z ^= z >> 31;
-    return (uint32_t)(z >> (64 - bits));
+    const uint32_t local = (uint32_t)(((cx & 3) << 4) | ((cy & 3) << 2) | (cz & 3));
+    return ((uint32_t)(z >> (64 - (bits - 6))) << 6) | local;
 }
 
 // Bucket of each active body's cell (inactive bodies: 2^bits, after every
@@ -230,8 +236,8 @@ k_cell_keys(int n, int bits, const double* __restrict__ px, const double* __rest
     uint32_t k = 1u << bits;
     if (act[i]) {
         double inv = h->inv_cell;
-        k = cell_bucket(cell_key(cell_coord(px[i], inv), cell_coord(py[i], inv),
-                                 cell_coord(pz[i], inv)), bits);
+        k = cell_bucket(cell_coord(px[i], inv), cell_coord(py[i], inv), cell_coord(pz[i], inv),
+                        bits);
     }
     keys[i] = k;
     vals[i] = i;
@@ -319,7 +325,7 @@ k_detect(int n, int bits, const uint32_t* __restrict__ keys, const int32_t* __re
     for (int ddx = -1; ddx <= 1; ddx++) {
         for (int ddy = -1; ddy <= 1; ddy++) {
             for (int ddz = -1; ddz <= 1; ddz++) {
-                const uint32_t bk = cell_bucket(cell_key(cx + ddx, cy + ddy, cz + ddz), bits);
+                const uint32_t bk = cell_bucket(cx + ddx, cy + ddy, cz + ddz, bits);
                 bool dup = false;
                 for (int q = 0; q < ns; q++) dup = dup || seen[q] == bk;
                 if (dup) continue;
