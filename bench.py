@@ -295,7 +295,8 @@ def disk(args, cmhip):
     soa = cmhip.generate_disk(n, seed=42, a_min=0.5, a_max=5.0, sigma_e=0.02, sigma_i=0.01,
                               m_min=1e-9, m_max=1e-6,
                               density=cmhip.ROCK_DENSITY_INFLATED)
-    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5)
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5,
+                                 leaf_capacity=args.disk_leaf_capacity)
     mode = {"merge": cmhip.COLLISION_MERGE_ONLY, "bounce": cmhip.COLLISION_BOUNCE_ONLY,
             "realistic": cmhip.COLLISION_REALISTIC}[args.collision_mode]
     be.configure_collisions(mode=mode)
@@ -619,6 +620,9 @@ def main():
                     help="time only the FP32 theta = 0.5 line (profiling runs)")
     ap.add_argument("--leaf-capacity", type=int, default=LEAF_CAPACITY,
                     help="bodies per leaf bucket of the Barnes-Hut lines")
+    ap.add_argument("--disk-leaf-capacity", type=int, default=8,
+                    help="bodies per leaf bucket of the disk workload's Barnes-Hut step "
+                         "(8 measured fastest of 8-32)")
     ap.add_argument("--group-size", type=int, default=GROUP_SIZE,
                     help="bodies per target group of the Barnes-Hut paths (1..64)")
     ap.add_argument("--train-epochs", type=int, default=20,
