@@ -324,7 +324,7 @@ def disk(args, cmhip):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f64", "data": "synthetic",
             "config": {"workload": f"disk_collisions_{args.collision_mode}", "n_bodies": n,
-                       "theta": 0.5,
+                       "theta": 0.5, "leaf_capacity": args.disk_leaf_capacity,
                        "softening": eps, "dt": dt, "seed": 42},
             "body_steps_per_s": n * args.steps / wall, "events": events, "merged": merged,
             "bodies_left": be.count}
