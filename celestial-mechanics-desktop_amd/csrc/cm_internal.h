@@ -105,6 +105,10 @@ struct Engine {
                                        // previous key, deepest level it opens
     uint32_t* tree_tcount = nullptr;   // (kMaxLevels + 1) x tree_tiles: nodes per tile
     int2* groups = nullptr;            // target groups: sorted body range [x, y), in body order
+    int2* gq = nullptr;                // the groups in traversal order (k_group_order)
+    int32_t* gcost = nullptr;          // per group in body order: work in the last traversal
+    int32_t* bcost = nullptr;          // per body (caller's index): work units of its group
+                                       // in the last traversal (orders the next one)
     int32_t* gmark = nullptr;          // per sorted body: end of the group starting there, or 0
     int32_t* gtile = nullptr;          // group counts / offsets per tile of 1024 bodies
     float* nd_quad = nullptr;          // 6 x node_cap second moments (learned correction)
@@ -190,7 +194,8 @@ void collide_free(Engine& e);
 // <= 10), else 64-bit.
 inline bool tree_keys32(const Engine& e) { return e.cfg.morton_levels <= 10; }
 int build_tree(Engine& e, const void** keys_out, const int32_t** order_out);
-int form_groups(Engine& e);
+// order: sorted position -> caller's index, from build_tree
+int form_groups(Engine& e, const int32_t* order);
 void mlp_free(Engine& e);
 int launch_pn(Engine& e, double softening);
 void pn_free(Engine& e);

@@ -8,7 +8,8 @@
  * reference engine's morton_code_kernel and radix sort
  * (engine/src/cuda/morton_kernel.cu:11-45,185-230), and the level-order
  * octree with its masses and centres of mass that replaces its insertion
- * build (engine/src/cuda/gpu_tree.cu:269-464).
+ * build (engine/src/cuda/gpu_tree.cu:269-464); and the target groups with
+ * the order in which the traversal takes them.
  */
 #ifndef CM_TESTING_H
 #define CM_TESTING_H
@@ -35,6 +36,20 @@ extern "C" {
 int32_t cm_test_tree(void* engine, int32_t node_cap, uint64_t* keys, int32_t* order,
                      double* box4, int32_t* n_active, int32_t* n_nodes, int32_t* info,
                      double* com_mass, int32_t* level_start);
+
+/* Builds the tree and the target groups for the current device state, as a
+ * force evaluation does before its traversal, and copies out:
+ *   groups[2 * g]  sorted body range [start, end) of each group, in body order;
+ *   queue[2 * g]   the same groups in traversal order: eighth q of them is
+ *                  XCD q's queue, its heavy groups (at least twice the mean
+ *                  work of the last traversal) first;
+ *   cost[count]    per body (caller's index): the work units of its group in
+ *                  the last traversal, which set that order (0 before any);
+ *   order[count]   original index of each sorted position;
+ *   *n_groups      the number of groups.
+ * At most cap groups are copied; any output pointer may be NULL. */
+int32_t cm_test_groups(void* engine, int32_t cap, int32_t* groups, int32_t* queue,
+                       int32_t* cost, int32_t* order, int32_t* n_groups);
 
 #ifdef __cplusplus
 }

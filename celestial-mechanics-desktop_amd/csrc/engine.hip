@@ -233,7 +233,7 @@ static void free_all(Engine& e) {
                     e.vals_b, e.hist, e.sorted, e.bbox_part, e.bbox_cnt, e.hdr,
                     e.nd_info, e.nd_cm, e.nd_sum, e.nd_parent, e.tree_dl, e.tree_tcount, e.pm4, e.nd_quad,
                     e.red, e.sample,
-                    e.groups, e.nd_box, e.gmark, e.gtile};
+                    e.groups, e.gq, e.bcost, e.gcost, e.nd_box, e.gmark, e.gtile};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     if (e.hdr_host) (void)hipHostFree(e.hdr_host);
     collide_free(e);
@@ -323,6 +323,9 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     alloc((void**)&e->tree_dl, n * sizeof(uchar2));
     alloc((void**)&e->tree_tcount, (size_t)(cm::kMaxLevels + 1) * cm::tree_tiles((int)n) * 4);
     alloc((void**)&e->groups, n * sizeof(int2));
+    alloc((void**)&e->gq, n * sizeof(int2));
+    alloc((void**)&e->bcost, n * 4);
+    alloc((void**)&e->gcost, n * 4);
     alloc((void**)&e->gmark, n * 4);
     alloc((void**)&e->gtile, ((n + 1023) / 1024 + 1) * 4);
     alloc((void**)&e->red, 64 * 1024 * 8);
@@ -338,6 +341,7 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     for (int k = 0; k < 3 * cm::kWindow; k++)
         if (hipEventCreate(&e->wev[k]) != hipSuccess) return fail(CM_ERR_HIP);
     if (hipMemsetAsync(e->hdr, 0, sizeof(cm::TreeHeader), e->stream) != hipSuccess ||
+        hipMemsetAsync(e->bcost, 0, n * 4, e->stream) != hipSuccess ||
         hipStreamSynchronize(e->stream) != hipSuccess)
         return fail(CM_ERR_HIP);
     // device address of cm_energy_noalloc's static scratch on this GPU

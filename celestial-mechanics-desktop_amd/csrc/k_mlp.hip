@@ -675,7 +675,7 @@ int32_t cm_mlp_collect(void* h, double softening, int32_t max_groups, uint64_t s
     rc = cm::build_tree(*e, &keys, &order);
     e->cfg.learned_correction = saved;
     if (rc != CM_OK) return rc;
-    rc = cm::form_groups(*e);
+    rc = cm::form_groups(*e, order);
     if (rc != CM_OK) return rc;
     CM_HIP_TRY(*e, hipMemcpyAsync(e->hdr_host, e->hdr, sizeof(cm::TreeHeader),
                                   hipMemcpyDeviceToHost, e->stream));

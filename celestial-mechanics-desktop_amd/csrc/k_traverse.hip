@@ -12,8 +12,11 @@
 //
 // Traversal (k_traverse): persistent wavefronts take groups from eight device
 // queues, one per XCD, each an eighth of the groups in body order (so each
-// XCD's L2 serves one compact region).  Each group walks the tree from the
-// root with an LDS stack,
+// XCD's L2 serves one compact region).  A queue runs first the groups that
+// did at least twice the mean work in the previous traversal (k_group_order);
+// the order decides only which wavefront computes a group and when, never its
+// result.  Each group walks the tree
+// from the root with an LDS stack,
 // 64 nodes per step (fewer when the stack is nearly full, so it never
 // overflows): nodes of zero mass are skipped (BarnesHutBackend.cs:402); a node
 // is accepted as a monopole when d2 > 0 and (4 hL) hL < theta^2 d2, d2 being
@@ -38,6 +41,7 @@
 
 #include "bh_device.h"
 #include "cm_internal.h"
+#include "cm_testing.h"
 
 namespace cm {
 
@@ -188,7 +192,8 @@ k_group_scan(int ntiles, TreeHeader* __restrict__ hdr, int32_t* __restrict__ tco
 __global__ void __launch_bounds__(kGroupTile)
 k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
              const int32_t* __restrict__ gmark, const int32_t* __restrict__ tcount,
-             int2* __restrict__ groups) {
+             const int32_t* __restrict__ order, const int32_t* __restrict__ bcost,
+             int2* __restrict__ groups, int32_t* __restrict__ gcost) {
     constexpr int kW = kGroupTile / kWave;
     __shared__ int s_w[kW];   // marks per wave of this tile
     __shared__ int s_b[kW];   // per wave: counts of the tiles before this one
@@ -229,6 +234,76 @@ k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
         for (int k = 0; k < w; k++) off += s_w[k];
         off += __popcll(m & ((1ull << lane) - 1ull));
         groups[off] = make_int2(i, e);
+        gcost[off] = bcost[order[i]];
+    }
+}
+
+// Traversal order.  Eighth q of the groups (in body order) is XCD q's queue.
+// Within it the heavy groups run first: those whose work in the previous
+// traversal, read at their first body from bcost (kept by the caller's body
+// index, so it survives the re-sort), is at least twice the mean over all
+// groups; the rest follow in body order.  In a dense region whose deepest
+// leaves hold many bodies a group does several times the mean work, and one
+// drawn late kept its wavefront running after the others had finished.  Only
+// the heavy groups move: a queue ordered entirely by descending work was
+// slower, because it ends on a run of light groups whose latency-bound walks
+// then have no arithmetic of other wavefronts to hide behind.  The partition
+// is stable, so each class keeps body order, and any bcost content (zeros for
+// a fresh engine: body order) gives a valid permutation; the order decides
+// which wavefront computes a group and when, never its result.
+constexpr int kHeavyFactor = 2;
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderWaves = kOrderThreads / kWave;
+
+// one block per eighth; gcost: the work of each group in body order (written
+// by k_group_emit)
+__global__ void __launch_bounds__(kOrderThreads)
+k_group_order(const TreeHeader* __restrict__ hdr, const int2* __restrict__ groups,
+              const int32_t* __restrict__ gcost, int2* __restrict__ gq) {
+    __shared__ long long s_w[kOrderWaves];
+    __shared__ int s_cnt[kOrderWaves][2];   // per wave: heavy, light; count, then base
+    const int ng = hdr->n_groups;
+    const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+    // mean work over all groups (each at least 1)
+    long long sum = 0;
+    for (int i = t; i < ng; i += kOrderThreads) sum += max(1, gcost[i]);
+    for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+    if (lane == 0) s_w[w] = sum;
+    __syncthreads();
+    long long total = 0;
+    for (int k = 0; k < kOrderWaves; k++) total += s_w[k];
+    const long long heavy = ng > 0 ? kHeavyFactor * total / ng : 0;
+    // wave w partitions a contiguous slice of the eighth
+    const int lo = (int)((long long)blockIdx.x * ng / kXcds);
+    const int len = (int)((long long)(blockIdx.x + 1) * ng / kXcds) - lo;
+    const int ws = lo + (int)((long long)len * w / kOrderWaves);
+    const int we = lo + (int)((long long)len * (w + 1) / kOrderWaves);
+    int nh = 0;
+    for (int i = ws + lane; i < we; i += kWave) nh += gcost[i] >= heavy ? 1 : 0;
+    for (int o = kWave / 2; o > 0; o >>= 1) nh += __shfl_xor(nh, o, kWave);
+    if (lane == 0) { s_cnt[w][0] = nh; s_cnt[w][1] = (we - ws) - nh; }
+    __syncthreads();
+    if (t == 0) {
+        // bases: the heavy groups of every slice, then the light ones
+        int run = lo;
+        for (int k = 0; k < 2; k++)
+            for (int u = 0; u < kOrderWaves; u++) {
+                const int c = s_cnt[u][k];
+                s_cnt[u][k] = run;
+                run += c;
+            }
+    }
+    __syncthreads();
+    int hb = s_cnt[w][0], lb = s_cnt[w][1];
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int b = ws; b < we; b += kWave) {
+        const int i = b + lane;
+        const bool ok = i < we;
+        const bool hv = ok && gcost[i] >= heavy;
+        const uint64_t mh = __ballot(hv), ml = __ballot(ok && !hv);
+        if (ok) gq[hv ? hb + __popcll(mh & lt) : lb + __popcll(ml & lt)] = groups[i];
+        hb += __popcll(mh);
+        lb += __popcll(ml);
     }
 }
 
@@ -238,7 +313,8 @@ struct TravArgs {
     const int4* info;
     const double4* cm;
     const int32_t* order;
-    const int2* groups;
+    const int2* groups;      // in traversal order (k_group_order)
+    int32_t* bcost;          // per body (caller's index): its group's work units
     const double* sorted;    // x, y, z, m, r blocks of `cap`
     int cap;
     int n;                   // bodies (active first, then inactive, in sorted order)
@@ -396,9 +472,10 @@ k_traverse(TravArgs a) {
     const int reserve = 8 * (a.levels + 2);
     unsigned long long n_cell = 0ull, n_body = 0ull, n_far = 0ull;
 
-    // groups are in body (Morton) order; eighth q of them is queue q, served
-    // first by the wavefronts of XCD q so that each XCD's L2 holds one compact
-    // region of the tree; a wavefront whose queue is empty helps the others
+    // eighth q of the groups (in body order; heavy ones first) is queue q,
+    // served first by the wavefronts of XCD q so that each XCD's L2 holds one
+    // compact region of the tree; a wavefront whose queue is empty helps the
+    // others
     const int xcd = blockIdx.x % kXcds;
     int qi = 0;
     for (;;) {
@@ -468,6 +545,10 @@ k_traverse(TravArgs a) {
         if constexpr (kSplit) {
             while (2 * kl * nmem <= kWave) { kl *= 2; klog++; }
         }
+        // the group's work in units of one list entry per lane (13 VALU
+        // instructions): entries / kl per flush, 14 per walk step, 5 per far
+        // flush; recorded for the next traversal's order
+        int wcost = 0;
         const int tl = lane % nmem, sub = lane / nmem;
         const bool worker = lane < kl * nmem;
         const int kw = kWave >> klog;                                   // W
@@ -588,6 +669,7 @@ k_traverse(TravArgs a) {
         };
 
         auto flush = [&](int cnt, bool cells) {
+            wcost += kSplit ? (cnt + kl - 1) >> klog : cnt;
             if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                 // the slots past cnt hold massless far entries, whose terms
                 // are exactly zero, so the loops below may run past cnt
@@ -779,6 +861,7 @@ k_traverse(TravArgs a) {
             }
             n_far += (unsigned long long)cnt;
             any_far = 1;
+            wcost += 5;
             __syncthreads();
           }
         };
@@ -787,6 +870,7 @@ k_traverse(TravArgs a) {
         if (lane == 0) s_stack[0] = 0;
         __syncthreads();
         while (S > 0) {
+            wcost += 14;
             int lim = (kCap - reserve - S) / 7;
             if (lim < 1) lim = 1;
             int take = S < kWave ? S : kWave;
@@ -934,6 +1018,7 @@ k_traverse(TravArgs a) {
         }
         if (member) {
             const int o = a.order[my];
+            a.bcost[o] = wcost;
             if constexpr (POT) {
                 a.rows[o] = 0.5 * (sm[my] * (acx + apx));
             } else if constexpr (PREC == CM_PREC_FP32_SPLIT) {
@@ -964,7 +1049,7 @@ k_traverse(TravArgs a) {
     }
 }
 
-int form_groups(Engine& e) {
+int form_groups(Engine& e, const int32_t* order) {
     const int n = e.count;
     const int gb = std::min(1024, std::max(1, (n + 255) / 256));
     const int tiles = std::max(1, (n + kGroupTile - 1) / kGroupTile);
@@ -976,7 +1061,9 @@ int form_groups(Engine& e) {
         hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e.stream, tiles, e.hdr,
                            e.gtile);
     hipLaunchKernelGGL(k_group_emit, dim3(tiles), dim3(kGroupTile), 0, e.stream, tiles,
-                       scanned, e.hdr, e.gmark, e.gtile, e.groups);
+                       scanned, e.hdr, e.gmark, e.gtile, order, e.bcost, e.groups, e.gcost);
+    hipLaunchKernelGGL(k_group_order, dim3(kXcds), dim3(kOrderThreads), 0, e.stream, e.hdr,
+                       e.groups, e.gcost, e.gq);
     CM_HIP_TRY(e, hipGetLastError());
     return CM_OK;
 }
@@ -998,7 +1085,7 @@ int launch_barnes_hut(Engine& e, double softening) {
                          "(cm_mlp_init / cm_mlp_set_params / cm_mlp_train)");
     int rc = build_tree(e, &keys, &order);
     if (rc != CM_OK) return rc;
-    rc = form_groups(e);
+    rc = form_groups(e, order);
     if (rc != CM_OK) return rc;
     if (e.detail_timing()) CM_HIP_TRY(e, hipEventRecord(e.ev[3], st));
     e.mark(1);
@@ -1007,7 +1094,8 @@ int launch_barnes_hut(Engine& e, double softening) {
     a.info = e.nd_info;
     a.cm = e.nd_cm;
     a.order = order;
-    a.groups = e.groups;
+    a.groups = e.gq;
+    a.bcost = e.bcost;
     a.sorted = e.sorted;
     a.cap = e.capacity;
     a.n = n;
@@ -1062,14 +1150,15 @@ int launch_bh_potential(Engine& e, double softening, double* rows) {
     const int32_t* order = nullptr;
     int rc = build_tree(e, &keys, &order);
     if (rc != CM_OK) return rc;
-    rc = form_groups(e);
+    rc = form_groups(e, order);
     if (rc != CM_OK) return rc;
     TravArgs a{};
     a.hdr = e.hdr;
     a.info = e.nd_info;
     a.cm = e.nd_cm;
     a.order = order;
-    a.groups = e.groups;
+    a.groups = e.gq;
+    a.bcost = e.bcost;
     a.sorted = e.sorted;
     a.cap = e.capacity;
     a.n = n;
@@ -1088,3 +1177,40 @@ int launch_bh_potential(Engine& e, double softening, double* rows) {
 }
 
 }  // namespace cm
+
+extern "C" int32_t cm_test_groups(void* h, int32_t cap, int32_t* groups, int32_t* queue,
+                                  int32_t* cost, int32_t* order, int32_t* n_groups) {
+    using namespace cm;
+    Engine* e = static_cast<Engine*>(h);
+    if (!e) return CM_ERR_NOT_INITIALIZED;
+    if (cap < 0) return set_error(*e, CM_ERR_INVALID_ARGUMENT, "negative cap");
+    CM_HIP_TRY(*e, hipSetDevice(e->device));
+    const int n = e->count;
+    if (n_groups) *n_groups = 0;
+    if (n == 0) return CM_OK;
+    const void* k = nullptr;
+    const int32_t* o = nullptr;
+    int rc = build_tree(*e, &k, &o);
+    if (rc != CM_OK) return rc;
+    rc = form_groups(*e, o);
+    if (rc != CM_OK) return rc;
+    TreeHeader hh;
+    CM_HIP_TRY(*e, hipMemcpyAsync(&hh, e->hdr, sizeof(hh), hipMemcpyDeviceToHost, e->stream));
+    CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
+    if (hh.overflow) return set_error(*e, CM_ERR_POOL_EXHAUSTED, "octree node pool exhausted");
+    if (n_groups) *n_groups = hh.n_groups;
+    const size_t m = (size_t)std::min(cap, hh.n_groups);
+    if (groups && m)
+        CM_HIP_TRY(*e, hipMemcpyAsync(groups, e->groups, m * sizeof(int2), hipMemcpyDeviceToHost,
+                                      e->stream));
+    if (queue && m)
+        CM_HIP_TRY(*e, hipMemcpyAsync(queue, e->gq, m * sizeof(int2), hipMemcpyDeviceToHost,
+                                      e->stream));
+    if (cost)
+        CM_HIP_TRY(*e, hipMemcpyAsync(cost, e->bcost, (size_t)n * 4, hipMemcpyDeviceToHost,
+                                      e->stream));
+    if (order)
+        CM_HIP_TRY(*e, hipMemcpyAsync(order, o, (size_t)n * 4, hipMemcpyDeviceToHost, e->stream));
+    CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
+    return CM_OK;
+}
