@@ -55,6 +55,7 @@ struct TreeHeader {
     unsigned long long pot_int[5];   // terms counted by potential evaluations (unused)
     int32_t n_groups;             // target groups formed for the traversal
     uint32_t work_next[8];        // per-XCD queues: next group of each eighth
+    unsigned long long group_work;   // sum over groups of max(1, last traversal's work)
     int32_t overflow_run;         // set by any evaluation of a run whose tree outgrew
                                   // the node pool; cleared by the host per API call
 };

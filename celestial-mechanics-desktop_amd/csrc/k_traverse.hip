@@ -198,10 +198,15 @@ k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
     __shared__ int s_w[kW];   // marks per wave of this tile
     __shared__ int s_b[kW];   // per wave: counts of the tiles before this one
     __shared__ int s_a[kW];   // per wave: counts of all tiles
+    __shared__ long long s_c[kW];   // per wave: work of its groups
     const int n = hdr->n_active;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int i = blockIdx.x * kGroupTile + threadIdx.x;
     const int e = i < n ? gmark[i] : 0;
+    // the group's work in the last traversal (at its first body), and the
+    // total over all groups (each at least 1) for k_group_order
+    const int c = e > 0 ? bcost[order[i]] : 0;
+    long long cw = e > 0 ? max(1, c) : 0;
     int before = 0, all = 0;
     if (!scanned) {
         for (int q = threadIdx.x; q < ntiles; q += kGroupTile) {
@@ -214,13 +219,20 @@ k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
             all += __shfl_xor(all, o, kWave);
         }
     }
+    for (int o = kWave / 2; o > 0; o >>= 1) cw += __shfl_xor(cw, o, kWave);
     const uint64_t m = __ballot(e > 0);
     if (lane == 0) {
         s_w[w] = __popcll(m);
         s_b[w] = before;
         s_a[w] = all;
+        s_c[w] = cw;
     }
     __syncthreads();
+    if (threadIdx.x == 0) {
+        long long bw = 0;
+        for (int k = 0; k < kW; k++) bw += s_c[k];
+        if (bw) atomicAdd(&hdr->group_work, (unsigned long long)bw);
+    }
     int base = 0;
     if (scanned) {
         base = tcount[blockIdx.x];
@@ -234,7 +246,7 @@ k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
         for (int k = 0; k < w; k++) off += s_w[k];
         off += __popcll(m & ((1ull << lane) - 1ull));
         groups[off] = make_int2(i, e);
-        gcost[off] = bcost[order[i]];
+        gcost[off] = c;
     }
 }
 
@@ -255,23 +267,16 @@ constexpr int kHeavyFactor = 2;
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderWaves = kOrderThreads / kWave;
 
-// one block per eighth; gcost: the work of each group in body order (written
-// by k_group_emit)
+// one block per eighth; gcost: the work of each group in body order, and
+// hdr->group_work their total (written by k_group_emit)
 __global__ void __launch_bounds__(kOrderThreads)
 k_group_order(const TreeHeader* __restrict__ hdr, const int2* __restrict__ groups,
               const int32_t* __restrict__ gcost, int2* __restrict__ gq) {
-    __shared__ long long s_w[kOrderWaves];
     __shared__ int s_cnt[kOrderWaves][2];   // per wave: heavy, light; count, then base
     const int ng = hdr->n_groups;
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-    // mean work over all groups (each at least 1)
-    long long sum = 0;
-    for (int i = t; i < ng; i += kOrderThreads) sum += max(1, gcost[i]);
-    for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
-    if (lane == 0) s_w[w] = sum;
-    __syncthreads();
-    long long total = 0;
-    for (int k = 0; k < kOrderWaves; k++) total += s_w[k];
+    // twice the mean work over all groups (each counted as at least 1)
+    const long long total = (long long)hdr->group_work;
     const long long heavy = ng > 0 ? kHeavyFactor * total / ng : 0;
     // wave w partitions a contiguous slice of the eighth
     const int lo = (int)((long long)blockIdx.x * ng / kXcds);
