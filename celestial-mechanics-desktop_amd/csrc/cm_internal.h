@@ -55,7 +55,6 @@ struct TreeHeader {
     unsigned long long pot_int[5];   // terms counted by potential evaluations (unused)
     int32_t n_groups;             // target groups formed for the traversal
     uint32_t work_next[8];        // per-XCD queues: next group of each eighth
-    unsigned long long group_work;   // sum over groups of max(1, last traversal's work)
     int32_t overflow_run;         // set by any evaluation of a run whose tree outgrew
                                   // the node pool; cleared by the host per API call
 };
@@ -108,6 +107,7 @@ struct Engine {
     int2* groups = nullptr;            // target groups: sorted body range [x, y), in body order
     int2* gq = nullptr;                // the groups in traversal order (k_group_order)
     int32_t* gcost = nullptr;          // per group in body order: work in the last traversal
+    long long* gwork = nullptr;        // per tile of 1024 bodies: its groups' work (>= 1 each)
     int32_t* bcost = nullptr;          // per body (caller's index): work units of its group
                                        // in the last traversal (orders the next one)
     int32_t* gmark = nullptr;          // per sorted body: end of the group starting there, or 0

@@ -233,7 +233,7 @@ static void free_all(Engine& e) {
                     e.vals_b, e.hist, e.sorted, e.bbox_part, e.bbox_cnt, e.hdr,
                     e.nd_info, e.nd_cm, e.nd_sum, e.nd_parent, e.tree_dl, e.tree_tcount, e.pm4, e.nd_quad,
                     e.red, e.sample,
-                    e.groups, e.gq, e.bcost, e.gcost, e.nd_box, e.gmark, e.gtile};
+                    e.groups, e.gq, e.bcost, e.gcost, e.gwork, e.nd_box, e.gmark, e.gtile};
     for (void* p : ptrs) if (p) (void)hipFree(p);
     if (e.hdr_host) (void)hipHostFree(e.hdr_host);
     collide_free(e);
@@ -326,6 +326,7 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     alloc((void**)&e->gq, n * sizeof(int2));
     alloc((void**)&e->bcost, n * 4);
     alloc((void**)&e->gcost, n * 4);
+    alloc((void**)&e->gwork, ((n + 1023) / 1024 + 1) * 8);
     alloc((void**)&e->gmark, n * 4);
     alloc((void**)&e->gtile, ((n + 1023) / 1024 + 1) * 4);
     alloc((void**)&e->red, 64 * 1024 * 8);

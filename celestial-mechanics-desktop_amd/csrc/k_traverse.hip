@@ -193,7 +193,8 @@ __global__ void __launch_bounds__(kGroupTile)
 k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
              const int32_t* __restrict__ gmark, const int32_t* __restrict__ tcount,
              const int32_t* __restrict__ order, const int32_t* __restrict__ bcost,
-             int2* __restrict__ groups, int32_t* __restrict__ gcost) {
+             int2* __restrict__ groups, int32_t* __restrict__ gcost,
+             long long* __restrict__ twork) {
     constexpr int kW = kGroupTile / kWave;
     __shared__ int s_w[kW];   // marks per wave of this tile
     __shared__ int s_b[kW];   // per wave: counts of the tiles before this one
@@ -203,8 +204,8 @@ k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int i = blockIdx.x * kGroupTile + threadIdx.x;
     const int e = i < n ? gmark[i] : 0;
-    // the group's work in the last traversal (at its first body), and the
-    // total over all groups (each at least 1) for k_group_order
+    // the group's work in the last traversal (at its first body), and this
+    // tile's total (each group at least 1) for k_group_order
     const int c = e > 0 ? bcost[order[i]] : 0;
     long long cw = e > 0 ? max(1, c) : 0;
     int before = 0, all = 0;
@@ -231,7 +232,7 @@ k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
     if (threadIdx.x == 0) {
         long long bw = 0;
         for (int k = 0; k < kW; k++) bw += s_c[k];
-        if (bw) atomicAdd(&hdr->group_work, (unsigned long long)bw);
+        twork[blockIdx.x] = bw;
     }
     int base = 0;
     if (scanned) {
@@ -267,16 +268,24 @@ constexpr int kHeavyFactor = 2;
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderWaves = kOrderThreads / kWave;
 
-// one block per eighth; gcost: the work of each group in body order, and
-// hdr->group_work their total (written by k_group_emit)
+// one block per eighth; gcost: the work of each group in body order, twork
+// the totals per tile of bodies (both written by k_group_emit)
 __global__ void __launch_bounds__(kOrderThreads)
-k_group_order(const TreeHeader* __restrict__ hdr, const int2* __restrict__ groups,
-              const int32_t* __restrict__ gcost, int2* __restrict__ gq) {
+k_group_order(int ntiles, const TreeHeader* __restrict__ hdr, const int2* __restrict__ groups,
+              const int32_t* __restrict__ gcost, const long long* __restrict__ twork,
+              int2* __restrict__ gq) {
     __shared__ int s_cnt[kOrderWaves][2];   // per wave: heavy, light; count, then base
+    __shared__ long long s_w[kOrderWaves];
     const int ng = hdr->n_groups;
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
     // twice the mean work over all groups (each counted as at least 1)
-    const long long total = (long long)hdr->group_work;
+    long long sum = 0;
+    for (int k = t; k < ntiles; k += kOrderThreads) sum += twork[k];
+    for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+    if (lane == 0) s_w[w] = sum;
+    __syncthreads();
+    long long total = 0;
+    for (int k = 0; k < kOrderWaves; k++) total += s_w[k];
     const long long heavy = ng > 0 ? kHeavyFactor * total / ng : 0;
     // wave w partitions a contiguous slice of the eighth
     const int lo = (int)((long long)blockIdx.x * ng / kXcds);
@@ -1066,9 +1075,10 @@ int form_groups(Engine& e, const int32_t* order) {
         hipLaunchKernelGGL(k_group_scan, dim3(1), dim3(1024), 0, e.stream, tiles, e.hdr,
                            e.gtile);
     hipLaunchKernelGGL(k_group_emit, dim3(tiles), dim3(kGroupTile), 0, e.stream, tiles,
-                       scanned, e.hdr, e.gmark, e.gtile, order, e.bcost, e.groups, e.gcost);
-    hipLaunchKernelGGL(k_group_order, dim3(kXcds), dim3(kOrderThreads), 0, e.stream, e.hdr,
-                       e.groups, e.gcost, e.gq);
+                       scanned, e.hdr, e.gmark, e.gtile, order, e.bcost, e.groups, e.gcost,
+                       e.gwork);
+    hipLaunchKernelGGL(k_group_order, dim3(kXcds), dim3(kOrderThreads), 0, e.stream, tiles,
+                       e.hdr, e.groups, e.gcost, e.gwork, e.gq);
     CM_HIP_TRY(e, hipGetLastError());
     return CM_OK;
 }

@@ -128,7 +128,6 @@ k_bbox_final(int nblocks, const double* __restrict__ part,
         hdr->n_far_int = 0ull;
         hdr->n_groups = 0;
         for (int q = 0; q < kXcds; q++) hdr->work_next[q] = 0u;
-        hdr->group_work = 0ull;
         nd_info[0] = make_int4(-1, 0, 0, na);
     }
 }
