@@ -23,7 +23,9 @@ constexpr int kPnWarnCap = 4096;   // strong-field pairs kept per evaluation
 
 struct PnBuf {
     cm_pn_warning* warn = nullptr;
-    int32_t* count = nullptr;
+    unsigned long long* count = nullptr;   // all strong-field pairs (64-bit: n^2 may pass 2^31)
+    int32_t* rowcnt = nullptr;   // strong-field pairs of each row i (capacity entries)
+    int32_t* rowoff = nullptr;   // their exclusive prefix sums, after an overflow
 };
 
 // PhysicalConstants (src/CelestialMechanics.Math/PhysicalConstants.cs:12-50):
@@ -39,7 +41,8 @@ struct PnArgs {
     const uint8_t* act;
     double *ax, *ay, *az;
     cm_pn_warning* warn;
-    int32_t* count;
+    unsigned long long* count;
+    int32_t* rowcnt;
 };
 
 struct PnTile {
@@ -60,6 +63,7 @@ __global__ void __launch_bounds__(kPnTile) k_pn(PnArgs a) {
     }
     const double vi2 = vxi * vxi + vyi * vyi + vzi * vzi;
     double sx = 0.0, sy = 0.0, sz = 0.0;
+    int nwarn = 0;   // this row's strong-field pairs
     for (int base = 0; base < a.n; base += kPnTile) {
         __syncthreads();
         const int j = base + threadIdx.x;
@@ -96,8 +100,9 @@ __global__ void __launch_bounds__(kPnTile) k_pn(PnArgs a) {
             const bool live = jj != i && !(mj <= 0.0) && !(vrel2 > a.max_vel2);
             const bool strong = dist < a.warn_factor * rs;
             if (live && strong) {
-                const int k = atomicAdd(a.count, 1);
-                if (k < kPnWarnCap) {
+                nwarn++;
+                const unsigned long long k = atomicAdd(a.count, 1ull);
+                if (k < (unsigned long long)kPnWarnCap) {
                     cm_pn_warning w;
                     w.body = i; w.other = jj; w.distance = dist; w.schwarzschild_radius = rs;
                     a.warn[k] = w;
@@ -128,6 +133,78 @@ __global__ void __launch_bounds__(kPnTile) k_pn(PnArgs a) {
         a.ay[i] += sy;
         a.az[i] += sz;
     }
+    if (i < a.n) a.rowcnt[i] = nwarn;
+}
+
+// More than kPnWarnCap strong-field pairs: the slots hold whichever pairs won
+// the atomic counter.  The two kernels below (each a no-op otherwise) replace
+// them by the first kPnWarnCap pairs of the reference's (i ascending,
+// j ascending) loop: the exclusive prefix sums of the per-row counts give
+// each row its first slot, and every row that starts below the cap recomputes
+// its strong-field pairs in j order with k_pn's arithmetic (the same doubles)
+// and writes them from there.
+constexpr int kPnScanThreads = 1024;
+
+__global__ void __launch_bounds__(kPnScanThreads)
+k_pn_row_offsets(int n, const unsigned long long* __restrict__ count,
+                 const int32_t* __restrict__ rowcnt, int32_t* __restrict__ rowoff) {
+    if (*count <= (unsigned long long)kPnWarnCap) return;
+    __shared__ long long s_w[kPnScanThreads / 64];
+    __shared__ long long s_carry;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += kPnScanThreads) {
+        const int i = base + t;
+        const long long v = i < n ? rowcnt[i] : 0;
+        long long x = v;   // inclusive scan within the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            const long long y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) s_w[w] = x;
+        __syncthreads();
+        long long before = s_carry;
+        for (int k = 0; k < w; k++) before += s_w[k];
+        // rows starting at or past the cap need no offset: clamp to it
+        if (i < n) rowoff[i] = (int32_t)min(before + x - v, (long long)kPnWarnCap);
+        __syncthreads();
+        if (t == 0) {
+            long long tot = 0;
+            for (int k = 0; k < kPnScanThreads / 64; k++) tot += s_w[k];
+            s_carry += tot;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kPnTile)
+k_pn_row_warnings(PnArgs a, const int32_t* __restrict__ rowoff) {
+    if (*a.count <= (unsigned long long)kPnWarnCap) return;
+    const int i = blockIdx.x * kPnTile + threadIdx.x;
+    if (i >= a.n || a.rowcnt[i] <= 0 || rowoff[i] < 0 || rowoff[i] >= kPnWarnCap) return;
+    const int first = rowoff[i];
+    const int room = min(a.rowcnt[i], kPnWarnCap - first);
+    const double xi = a.px[i], yi = a.py[i], zi = a.pz[i];
+    const double vxi = a.vx[i], vyi = a.vy[i], vzi = a.vz[i];
+    const double mi = a.m[i];
+    int k = 0;
+    for (int j = 0; j < a.n && k < room; j++) {
+        const double mj = a.m[j];
+        const double dx = a.px[j] - xi, dy = a.py[j] - yi, dz = a.pz[j] - zi;
+        const double dist2 = dx * dx + dy * dy + dz * dz + a.eps2;
+        const double dist = sqrt(dist2);
+        const double dvx = vxi - a.vx[j], dvy = vyi - a.vy[j], dvz = vzi - a.vz[j];
+        const double vrel2 = dvx * dvx + dvy * dvy + dvz * dvz;
+        const double rs = a.rs_factor * (mi + mj);
+        const bool live = j != i && !(mj <= 0.0) && !(vrel2 > a.max_vel2);
+        if (live && dist < a.warn_factor * rs) {
+            cm_pn_warning w;
+            w.body = i; w.other = j; w.distance = dist; w.schwarzschild_radius = rs;
+            a.warn[first + k] = w;
+            k++;
+        }
+    }
 }
 
 static PnBuf* pn_buf(Engine& e) { return static_cast<PnBuf*>(e.pn_buf); }
@@ -137,6 +214,8 @@ void pn_free(Engine& e) {
     if (!b) return;
     if (b->warn) (void)hipFree(b->warn);
     if (b->count) (void)hipFree(b->count);
+    if (b->rowcnt) (void)hipFree(b->rowcnt);
+    if (b->rowoff) (void)hipFree(b->rowoff);
     delete b;
     e.pn_buf = nullptr;
 }
@@ -146,12 +225,14 @@ static int pn_ensure(Engine& e) {
     PnBuf* b = new (std::nothrow) PnBuf();
     if (!b) return set_error(e, CM_ERR_OUT_OF_MEMORY, "1PN buffers");
     e.pn_buf = b;
+    const size_t rows = (size_t)std::max(e.capacity, 1) * 4;
     if (hipMalloc(&b->warn, sizeof(cm_pn_warning) * kPnWarnCap) != hipSuccess ||
-        hipMalloc(&b->count, 4) != hipSuccess) {
+        hipMalloc(&b->count, 8) != hipSuccess || hipMalloc(&b->rowcnt, rows) != hipSuccess ||
+        hipMalloc(&b->rowoff, rows) != hipSuccess) {
         pn_free(e);
         return set_error(e, CM_ERR_OUT_OF_MEMORY, "1PN buffers");
     }
-    CM_HIP_TRY(e, hipMemsetAsync(b->count, 0, 4, e.stream));
+    CM_HIP_TRY(e, hipMemsetAsync(b->count, 0, 8, e.stream));
     return CM_OK;
 }
 
@@ -160,7 +241,7 @@ int launch_pn(Engine& e, double softening) {
     int rc = pn_ensure(e);
     if (rc != CM_OK) return rc;
     PnBuf* b = pn_buf(e);
-    CM_HIP_TRY(e, hipMemsetAsync(b->count, 0, 4, e.stream));
+    CM_HIP_TRY(e, hipMemsetAsync(b->count, 0, 8, e.stream));
     if (e.count == 0) return CM_OK;
     const double c_sim = kCSi * kTimeUnit / kAu;
     const double c_sim2 = c_sim * c_sim;
@@ -179,8 +260,12 @@ int launch_pn(Engine& e, double softening) {
     a.ax = e.f[AX]; a.ay = e.f[AY]; a.az = e.f[AZ];
     a.warn = b->warn;
     a.count = b->count;
+    a.rowcnt = b->rowcnt;
     const int nb = (e.count + kPnTile - 1) / kPnTile;
     hipLaunchKernelGGL(k_pn, dim3(nb), dim3(kPnTile), 0, e.stream, a);
+    hipLaunchKernelGGL(k_pn_row_offsets, dim3(1), dim3(kPnScanThreads), 0, e.stream, a.n,
+                       b->count, b->rowcnt, b->rowoff);
+    hipLaunchKernelGGL(k_pn_row_warnings, dim3(nb), dim3(kPnTile), 0, e.stream, a, b->rowoff);
     CM_HIP_TRY(e, hipGetLastError());
     return CM_OK;
 }
@@ -213,7 +298,7 @@ int32_t cm_set_post_newtonian(void* h, const cm_pn_config* cfg) {
     if (cfg->enabled) return cm::pn_ensure(*e);
     // no evaluation adds the correction any more: forget the last warnings
     if (cm::PnBuf* b = cm::pn_buf(*e))
-        CM_HIP_TRY(*e, hipMemsetAsync(b->count, 0, 4, e->stream));
+        CM_HIP_TRY(*e, hipMemsetAsync(b->count, 0, 8, e->stream));
     return CM_OK;
 }
 
@@ -226,10 +311,10 @@ int32_t cm_pn_warnings(void* h, cm_pn_warning* out, int32_t cap, int32_t* n_tota
     cm::PnBuf* b = cm::pn_buf(*e);
     if (!b) return CM_OK;
     CM_HIP_TRY(*e, hipSetDevice(e->device));
-    int32_t total = 0;
-    CM_HIP_TRY(*e, hipMemcpyAsync(&total, b->count, 4, hipMemcpyDeviceToHost, e->stream));
+    unsigned long long total = 0;
+    CM_HIP_TRY(*e, hipMemcpyAsync(&total, b->count, 8, hipMemcpyDeviceToHost, e->stream));
     CM_HIP_TRY(*e, hipStreamSynchronize(e->stream));
-    const int kept = std::min(total, cm::kPnWarnCap);
+    const int kept = (int)std::min(total, (unsigned long long)cm::kPnWarnCap);
     std::vector<cm_pn_warning> w((size_t)kept);
     if (kept > 0) {
         CM_HIP_TRY(*e, hipMemcpyAsync(w.data(), b->warn, sizeof(cm_pn_warning) * kept,
@@ -241,7 +326,7 @@ int32_t cm_pn_warnings(void* h, cm_pn_warning* out, int32_t cap, int32_t* n_tota
         return x.body != y.body ? x.body < y.body : x.other < y.other;
     });
     for (int k = 0; k < std::min(kept, (int)cap); k++) out[k] = w[k];
-    *n_total = total;
+    *n_total = (int32_t)std::min(total, (unsigned long long)INT32_MAX);
     return CM_OK;
 }
 

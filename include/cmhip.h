@@ -362,11 +362,10 @@ int32_t cm_default_pn_config(cm_pn_config* cfg);
 int32_t cm_set_post_newtonian(void* engine, const cm_pn_config* cfg);
 /* OnSchwarzschildProximityWarning (PostNewtonian1Correction.cs:47,124-129):
  * the pairs the last force evaluation skipped as strong-field, in (i, j)
- * order.  Up to 4096 pairs per evaluation are kept; *n_total counts all.
- * When more than 4096 occur, the kept ones are the first 4096 the device
- * stored (a subset that can differ from run to run, not the first 4096 of
- * the reference's (i, j) loop), still returned sorted by (i, j); the
- * corrections themselves do not depend on it. */
+ * order.  Up to 4096 pairs per evaluation are kept: the first 4096 of the
+ * reference's (i ascending, j ascending) loop, also when more occur (then a
+ * second pass rewrites the kept slots in loop order); *n_total counts all
+ * (saturated at INT32_MAX). */
 typedef struct {
     int32_t body, other;
     double distance, schwarzschild_radius;

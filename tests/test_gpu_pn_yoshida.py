@@ -65,6 +65,37 @@ def test_pn_direct_bit_exact_with_warnings():
     np.testing.assert_array_equal([w[3] for w in got], [w[3] for w in warns])
 
 
+def test_pn_warnings_past_the_cap_are_the_first_in_loop_order():
+    """More strong-field pairs than the 4096 kept: the kept ones are the first
+    4096 of the reference's (i, j) loop (PostNewtonian1Correction.cs:124-129
+    raises them in that order), with the oracle's distances and radii bit for
+    bit, on every evaluation; the total counts them all."""
+    n = 400
+    soa = relativistic_cluster(n, seed=21)
+    rng = np.random.default_rng(21)
+    core = np.arange(100, 230)   # 130 heavy bodies inside 1e-4 AU: every live pair is strong
+    for f in ("pos_x", "pos_y", "pos_z"):
+        getattr(soa, f)[core] = 0.2 + rng.uniform(0.0, 1e-4, core.size)
+    soa.mass[core] = 3000.0
+    soa.mass[core[::17]] = 0.0           # massless sources are skipped before the test
+    soa.is_active[core[5::29]] = 0
+    s = soa_to_state(soa)
+    ref, warns, nw = _pn_ref(s, O.direct_gk(s, EPS))
+    assert nw > 3 * 4096
+    be = cmhip.HipPhysicsBackend(n, kind=cmhip.BACKEND_DIRECT)
+    be.set_post_newtonian(True)
+    for rep in range(2):
+        out = soa.copy()
+        be.compute_forces(out, EPS)
+        _eq(_acc(out), ref)
+        got, total = be.pn_warnings(cap=4096)
+        assert total == nw, rep
+        assert [(w[0], w[1]) for w in got] == [(w[0], w[1]) for w in warns[:4096]], rep
+        np.testing.assert_array_equal([w[2] for w in got], [w[2] for w in warns[:4096]])
+        np.testing.assert_array_equal([w[3] for w in got], [w[3] for w in warns[:4096]])
+    be.close()
+
+
 def test_pn_barnes_hut_fp64_bit_exact():
     n = 3000
     soa = relativistic_cluster(n, seed=12, massless=False)
