@@ -24,9 +24,11 @@
 // hL = h 2^-level (BarnesHutBackend.cs:431-433 with the body replaced by the
 // nearest point of the box); otherwise a leaf contributes its bodies and an
 // internal node pushes its children.  Accepted nodes and leaf bodies go to two
-// 64-entry LDS interaction lists; a full list is staged once (one entry per
-// lane) and every member lane sums all its entries (FP32 split: a group of at
-// most 32 bodies spreads each target over several lanes, see below).
+// LDS interaction lists (64 entries; 128 in the FP32 split kernel, whose far
+// cells go to a third); a full list is gathered and staged once (one entry per
+// lane and 64-entry half) and every member lane sums all its entries (FP32
+// split: a group of at most 32 bodies spreads each target over several
+// lanes, see below).
 // FP64 (CM_PREC_FP64): monopole terms use the reference's monopole arithmetic
 // (BarnesHutBackend.cs:438-445), body terms GravityKernel's
 // (GravityKernel.cs:9-33); each lane keeps the two sums apart and adds them at
@@ -431,21 +433,28 @@ __device__ __forceinline__ float gk_coeff_f(float dist_sq, float m, float r,
     return m * inv * inv * inv;
 }
 
-template <int PREC>
-struct Staging { double4 v[kWave]; double r[kWave]; int id[kWave]; };
-template <>
-struct Staging<CM_PREC_FP32_SPLIT> { float4 v[kWave]; float r[kWave]; int id[kWave]; };
+template <int PREC, int N>
+struct Staging { double4 v[N]; double r[N]; int id[N]; };
+template <int N>
+struct Staging<CM_PREC_FP32_SPLIT, N> { float4 v[N]; float r[N]; int id[N]; };
 
 // Occupancy of the traversal: the FP32 split kernel is held to the register
 // budget of four wavefronts per SIMD (16 per CU, as many as its LDS allows;
 // five or six per SIMD with a smaller stack measured slower); the others
-// take what they need.  All keep the stack the CPU restatement of this
-// traversal models (kStackCap), which fixes the order of the FP64 sums.
+// take what they need.  The FP64 paths keep the stack the CPU restatement of
+// this traversal models (kStackCap), which fixes the order of their sums.
+// The FP32 split kernel, whose sums are judged by tolerance, takes a stack of
+// 1,024 entries (the CPU model of the walk at N = 2^20: 0.7 % more walk
+// steps than with 1,536, deepest stack 856) and spends the LDS saved on
+// interaction lists of 128 entries (bodies, accepted cells and far cells):
+// each gather of list entries from global memory then feeds 128 entries
+// instead of 64, halving those round trips.
 template <int PREC, bool LEARNED>
 struct TravShape {
     static constexpr bool split = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
-    static constexpr int stack = kStackCap;
+    static constexpr int stack = split ? 1024 : kStackCap;
     static constexpr int waves = split ? 4 : 1;   // minimum per SIMD
+    static constexpr int list = split ? 2 * kWave : kWave;   // interaction-list capacity
 };
 
 // POT (FP64 only): the same walk sums the potential phi_i = -sum m / sqrt(d^2
@@ -458,16 +467,17 @@ __global__ void __launch_bounds__(kWave, (TravShape<PREC, LEARNED>::waves))
 k_traverse(TravArgs a) {
     static_assert(!POT || (PREC == CM_PREC_FP64 && !LEARNED), "potential mode is FP64");
     constexpr int kCap = TravShape<PREC, LEARNED>::stack;
+    constexpr int kPL = TravShape<PREC, LEARNED>::list;
     __shared__ int s_stack[kCap];
-    __shared__ int s_cell[2 * kWave];
+    __shared__ int s_cell[kPL + kWave];   // a walk step adds at most 64
     constexpr bool kFar = PREC == CM_PREC_FP32_SPLIT && !LEARNED;
-    __shared__ int s_far[kFar ? 2 * kWave : 1];
-    __shared__ int s_part[kWave];
+    __shared__ int s_far[kFar ? kPL + kWave : 1];
+    __shared__ int s_part[kPL];
     __shared__ int s_lpref[kWave];
     __shared__ int s_lstart[kWave];
     __shared__ int s_mark[kWave];
     __shared__ int s_g;
-    __shared__ Staging<PREC> st;
+    __shared__ Staging<PREC, kPL> st;
     // MLP inputs, bf16 [cell][16]; read as 8-byte operands
     __shared__ __align__(16) uint16_t s_mx[LEARNED ? kWave * kMlpIn : 4];
     __shared__ float s_my[LEARNED ? kWave * kMlpUsed : 1];   // MLP outputs per cell
@@ -551,8 +561,10 @@ k_traverse(TravArgs a) {
         // list for target l % nmem, and the kl partial sums are added in sub
         // order at the end.  The staged list is stored so that each sub's
         // entries are contiguous: entry j sits in slot (j % kl) * W + j / kl,
-        // W = 64 / kl, so lane sub reads slots sub * W + t, t = 0, 1, ...
-        // The FP64 baseline (FP64_FAST) splits small groups the same way.
+        // W = C / kl for a list of C slots (64 for cells, 128 for bodies),
+        // so lane sub reads slots sub * W + t, t = 0, 1, ...
+        // The FP64 baseline (FP64_FAST) splits small groups the same way
+        // (its lists have 64 slots).
         constexpr bool kSplit = (PREC == CM_PREC_FP32_SPLIT && !LEARNED) ||
                                 (PREC == CM_PREC_FP64_FAST && !POT);
         int kl = 1, klog = 0;
@@ -565,9 +577,6 @@ k_traverse(TravArgs a) {
         int wcost = 0;
         const int tl = lane % nmem, sub = lane / nmem;
         const bool worker = lane < kl * nmem;
-        const int kw = kWave >> klog;                                   // W
-        const int slot = ((lane & (kl - 1)) << (6 - klog)) | (lane >> klog);
-        const int rbase = worker ? sub * kw : 0;
         float txw = txf, tyw = tyf, tzw = tzf;
         double xw = xi, yw = yi, zw = zi;   // FP64_FAST: the lane's target
         if constexpr (kSplit) {
@@ -684,41 +693,76 @@ k_traverse(TravArgs a) {
 
         auto flush = [&](int cnt, bool cells) {
             wcost += kSplit ? (cnt + kl - 1) >> klog : cnt;
-            if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                // the slots past cnt hold massless far entries, whose terms
-                // are exactly zero, so the loops below may run past cnt
-                if (lane >= cnt) st.v[slot] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
-            } else if constexpr (kSplit) {
-                if (lane >= cnt) st.v[slot] = make_double4(1e18, 1e18, 1e18, 0.0);
+            // the list's kPL slots; W = kPL / kl
+            const int wlog = (kPL == kWave ? 6 : 7) - klog;
+            const int kw = kPL >> klog;
+            const int rbase = worker ? sub * kw : 0;
+            // lists of 128 entries: both halves' gathers are issued before
+            // either is converted, so one round trip serves them
+            double4 g[kPL / kWave];
+            float gr[kPL / kWave];
+#pragma unroll
+            for (int hh = 0; hh < kPL / kWave; hh++) {
+                const int j = hh * kWave + lane;
+                if (j < cnt) {
+                    if (cells) {
+                        g[hh] = a.cm[s_cell[j]];
+                    } else {
+                        const int q = s_part[j];
+                        g[hh] = make_double4(sx[q], sy[q], sz[q], sm[q]);
+                        gr[hh] = 0.0f;
+                        if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                            if (shell) gr[hh] = (float)sr[q];
+                        }
+                    }
+                }
             }
-            if (lane < cnt) {
-                if (cells) {
-                    double4 v = a.cm[s_cell[lane]];
-                    if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                        st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
-                                                 (float)(v.z - czg), (float)v.w);
+#pragma unroll
+            for (int hh = 0; hh < kPL / kWave; hh++) {
+                const int j = hh * kWave + lane;
+                const int slot = kSplit ? (((j & (kl - 1)) << wlog) | (j >> klog)) : j;
+                if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                    // the slots past cnt hold massless far entries, whose terms
+                    // are exactly zero, so the loops below may run past cnt
+                    if (j >= cnt) st.v[slot] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+                } else if constexpr (kSplit) {
+                    if (j >= cnt) st.v[slot] = make_double4(1e18, 1e18, 1e18, 0.0);
+                }
+                if (j < cnt) {
+                    if (cells) {
+                        const double4 v = g[hh];
+                        if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                            st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
+                                                     (float)(v.z - czg), (float)v.w);
+                        } else {
+                            st.v[slot] = v;
+                        }
                     } else {
-                        st.v[kSplit ? slot : lane] = v;
+                        const double4 v = g[hh];
+                        if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                            st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
+                                                     (float)(v.z - czg), (float)v.w);
+                            st.r[j] = gr[hh];
+                        } else {
+                            const int q = s_part[j];
+                            st.v[slot] = v;
+                            st.r[j] = shell ? sr[q] : 0.0;
+                        }
+                        st.id[j] = s_part[j];
                     }
-                } else {
-                    int q = s_part[lane];
-                    if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                        st.v[slot] = make_float4((float)(sx[q] - cxg), (float)(sy[q] - cyg),
-                                                 (float)(sz[q] - czg), (float)sm[q]);
-                        st.r[lane] = shell ? (float)sr[q] : 0.0f;
-                    } else {
-                        st.v[kSplit ? slot : lane] = make_double4(sx[q], sy[q], sz[q], sm[q]);
-                        st.r[lane] = shell ? sr[q] : 0.0;
-                    }
-                    st.id[lane] = q;
                 }
             }
             __syncthreads();
             if (cells) {
                 n_cell += (unsigned long long)cnt * nmem;
             } else {
-                uint64_t selfm = __ballot(lane < cnt && s_part[lane] >= s && s_part[lane] < e);
-                n_body += (unsigned long long)cnt * nmem - (unsigned long long)__popcll(selfm);
+#pragma unroll
+                for (int hh = 0; hh < kPL / kWave; hh++) {
+                    const int j = hh * kWave + lane;
+                    uint64_t selfm = __ballot(j < cnt && s_part[j] >= s && s_part[j] < e);
+                    n_body += (unsigned long long)min(max(cnt - hh * kWave, 0), kWave) * nmem -
+                              (unsigned long long)__popcll(selfm);
+                }
             }
             if (kSplit ? worker : member) {
                 if constexpr (PREC == CM_PREC_FP32_SPLIT) {
@@ -837,11 +881,18 @@ k_traverse(TravArgs a) {
             __syncthreads();
         };
 
-        // one far cell per lane: add its expansion coefficients
+        // one far cell per lane and half list: add its expansion coefficients
+        // (both halves' cells are loaded first, one round trip for 128)
         auto far_flush = [&](int cnt) {
           if constexpr (kFar) {
-            if (lane < cnt) {
-                const double4 v = a.cm[s_far[lane]];
+            double4 fv[kPL / kWave];
+#pragma unroll
+            for (int hh = 0; hh < kPL / kWave; hh++)
+                if (hh * kWave + lane < cnt) fv[hh] = a.cm[s_far[hh * kWave + lane]];
+#pragma unroll
+            for (int hh = 0; hh < kPL / kWave; hh++) {
+              if (hh * kWave + lane < cnt) {
+                const double4 v = fv[hh];
                 const float rx = (float)(v.x - cxg), ry = (float)(v.y - cyg),
                             rz = (float)(v.z - czg), M = (float)v.w;
                 const float s2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry,
@@ -872,10 +923,11 @@ k_traverse(TravArgs a) {
                 f2zzx = __builtin_fmaf(qz, rx, f2zzx);
                 f2zzy = __builtin_fmaf(qz, ry, f2zzy);
                 f2xyz = __builtin_fmaf(w * rx, ry * rz, f2xyz);
+              }
             }
             n_far += (unsigned long long)cnt;
             any_far = 1;
-            wcost += 5;
+            wcost += 5 * (kPL / kWave);
             __syncthreads();
           }
         };
@@ -928,20 +980,20 @@ k_traverse(TravArgs a) {
             s_lstart[lane] = nd.z;
             __syncthreads();
             if constexpr (kFar) {
-                if (nfar >= kWave) {
-                    far_flush(kWave);
-                    int rest = nfar - kWave;
-                    int v = lane < rest ? s_far[kWave + lane] : 0;
+                if (nfar >= kPL) {
+                    far_flush(kPL);
+                    int rest = nfar - kPL;
+                    int v = lane < rest ? s_far[kPL + lane] : 0;
                     __syncthreads();
                     if (lane < rest) s_far[lane] = v;
                     nfar = rest;
                     __syncthreads();
                 }
             }
-            while (ncell >= kWave) {
-                flush(kWave, true);
-                int rest = ncell - kWave;
-                int v = lane < rest ? s_cell[kWave + lane] : 0;
+            while (ncell >= kPL) {
+                flush(kPL, true);
+                int rest = ncell - kPL;
+                int v = lane < rest ? s_cell[kPL + lane] : 0;
                 __syncthreads();
                 if (lane < rest) s_cell[lane] = v;
                 ncell = rest;
@@ -954,7 +1006,8 @@ k_traverse(TravArgs a) {
             // it carried in) gives every position its leaf
             int done = 0, carry = -1;
             while (done < ltot) {
-                int room = kWave - npart;
+                // a window covers at most 64 positions (one per lane)
+                const int room = min(kPL - npart, kWave);
                 int tk = ltot - done < room ? ltot - done : room;
                 s_mark[lane] = -1;
                 __syncthreads();
@@ -972,7 +1025,7 @@ k_traverse(TravArgs a) {
                 npart += tk;
                 done += tk;
                 __syncthreads();
-                if (npart == kWave) { flush(kWave, false); npart = 0; }
+                if (npart == kPL) { flush(kPL, false); npart = 0; }
             }
         }
         if (ncell > 0) flush(ncell, true);
