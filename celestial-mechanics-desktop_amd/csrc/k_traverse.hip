@@ -561,10 +561,9 @@ k_traverse(TravArgs a) {
         // list for target l % nmem, and the kl partial sums are added in sub
         // order at the end.  The staged list is stored so that each sub's
         // entries are contiguous: entry j sits in slot (j % kl) * W + j / kl,
-        // W = C / kl for a list of C slots (64 for cells, 128 for bodies),
+        // W = C / kl for a list of C slots (128; 64 in the FP64 kernels),
         // so lane sub reads slots sub * W + t, t = 0, 1, ...
-        // The FP64 baseline (FP64_FAST) splits small groups the same way
-        // (its lists have 64 slots).
+        // The FP64 baseline (FP64_FAST) splits small groups the same way.
         constexpr bool kSplit = (PREC == CM_PREC_FP32_SPLIT && !LEARNED) ||
                                 (PREC == CM_PREC_FP64_FAST && !POT);
         int kl = 1, klog = 0;
@@ -572,8 +571,8 @@ k_traverse(TravArgs a) {
             while (2 * kl * nmem <= kWave) { kl *= 2; klog++; }
         }
         // the group's work in units of one list entry per lane (13 VALU
-        // instructions): entries / kl per flush, 14 per walk step, 5 per far
-        // flush; recorded for the next traversal's order
+        // instructions): entries / kl per flush, 14 per walk step, 5 per 64
+        // far cells; recorded for the next traversal's order
         int wcost = 0;
         const int tl = lane % nmem, sub = lane / nmem;
         const bool worker = lane < kl * nmem;
