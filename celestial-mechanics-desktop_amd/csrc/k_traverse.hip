@@ -728,26 +728,19 @@ k_traverse(TravArgs a) {
                     if (j >= cnt) st.v[slot] = make_double4(1e18, 1e18, 1e18, 0.0);
                 }
                 if (j < cnt) {
-                    if (cells) {
-                        const double4 v = g[hh];
-                        if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                            st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
-                                                     (float)(v.z - czg), (float)v.w);
-                        } else {
-                            st.v[slot] = v;
-                        }
+                    // a cell's centre of mass or a body: (x, y, z, m)
+                    const double4 v = g[hh];
+                    if constexpr (PREC == CM_PREC_FP32_SPLIT) {
+                        st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
+                                                 (float)(v.z - czg), (float)v.w);
                     } else {
-                        const double4 v = g[hh];
-                        if constexpr (PREC == CM_PREC_FP32_SPLIT) {
-                            st.v[slot] = make_float4((float)(v.x - cxg), (float)(v.y - cyg),
-                                                     (float)(v.z - czg), (float)v.w);
-                            st.r[j] = gr[hh];
-                        } else {
-                            const int q = s_part[j];
-                            st.v[slot] = v;
-                            st.r[j] = shell ? sr[q] : 0.0;
-                        }
-                        st.id[j] = s_part[j];
+                        st.v[slot] = v;
+                    }
+                    if (!cells) {
+                        const int q = s_part[j];
+                        if constexpr (PREC == CM_PREC_FP32_SPLIT) st.r[j] = gr[hh];
+                        else st.r[j] = shell ? sr[q] : 0.0;
+                        st.id[j] = q;
                     }
                 }
             }
