@@ -78,3 +78,28 @@ def test_reused_engine_verlet_runs_equal_fresh_engines():
                                           err_msg=f"run {k}: {f}")
         fresh.close()
     be.close()
+
+
+@pytest.mark.parametrize("path", ["fp64", "fp32_far"])
+def test_force_evaluations_before_steps_leave_the_trajectory_unchanged(path):
+    """bench.py evaluates the forces of the resident state for a while before
+    stepping (to bring the GPU's clocks up): a step's first kick reads the
+    previous accelerations (OldAcc), which only a step rotates in, so the
+    state after the steps is bit-identical with or without those evaluations."""
+    n, steps = 30000, 3
+    soa = _state(n, 31, 0.05)
+    runs = []
+    for extra in (0, 4):
+        be = cmhip.HipPhysicsBackend(n, **KW[path])
+        be.upload(soa)
+        for _ in range(extra):
+            be.compute_forces_resident(EPS)
+        be.step(EPS, 1e-3, steps)
+        out = cmhip.BodySoA(n)
+        be.download(out)
+        runs.append(out)
+        be.close()
+    for f in ("pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z", "acc_x", "acc_y", "acc_z",
+              "old_acc_x", "old_acc_y", "old_acc_z"):
+        np.testing.assert_array_equal(getattr(runs[0], f)[:n], getattr(runs[1], f)[:n],
+                                      err_msg=f"{path} {f}")
