@@ -954,8 +954,8 @@ int build_tree(Engine& e, const void** keys_out, const int32_t** order_out) {
                        e.node_cap, tstride, e.hdr, e.tree_dl, e.tree_tcount, e.sorted,
                        e.capacity, e.nd_info, e.nd_parent, e.nd_cm, e.nd_sum, e.nd_box,
                        e.cfg.learned_correction ? e.nd_quad : nullptr);
-    // deepest internal nodes live at level levels - 1
-    // deepest internal nodes live at level levels - 1
+    // deepest internal nodes live at level levels - 1 (one launch per level: a
+    // single block for the top four levels measured 25 us slower per build)
     for (int L = levels - 1; L >= 0; L--) {
         double maxsz = std::min((double)n, L < 20 ? std::ldexp(1.0, 3 * L) : 1e18);
         int g = (int)std::min(2048.0, std::max(1.0, std::ceil(maxsz / 32)));
