@@ -57,6 +57,11 @@ struct TreeHeader {
     uint32_t work_next[8];        // per-XCD queues: next group of each eighth
     int32_t overflow_run;         // set by any evaluation of a run whose tree outgrew
                                   // the node pool; cleared by the host per API call
+    // per-XCD queues of the traversal (k_group_order): entries
+    // [qbase[q], qend[q]) of the queue array; a cache line of their own, away
+    // from the atomically advanced work_next
+    alignas(128) int32_t qbase[8];
+    int32_t qend[8];
 };
 
 // Workgroups of one launch are dispatched round-robin over the 8 XCDs of an
@@ -105,7 +110,9 @@ struct Engine {
                                        // previous key, deepest level it opens
     uint32_t* tree_tcount = nullptr;   // (kMaxLevels + 1) x tree_tiles: nodes per tile
     int2* groups = nullptr;            // target groups: sorted body range [x, y), in body order
-    int2* gq = nullptr;                // the groups in traversal order (k_group_order)
+    int4* gq = nullptr;                // traversal queue entries (k_group_order): the group's
+                                       // sorted range [x, y) and the targets [z, w) it computes
+    int gq_cap = 0;                    // entries gq holds
     int32_t* gcost = nullptr;          // per group in body order: work in the last traversal
     long long* gwork = nullptr;        // per tile of 1024 bodies: its groups' work (>= 1 each)
     int32_t* bcost = nullptr;          // per body (caller's index): work units of its group

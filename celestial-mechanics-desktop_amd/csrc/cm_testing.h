@@ -40,16 +40,20 @@ int32_t cm_test_tree(void* engine, int32_t node_cap, uint64_t* keys, int32_t* or
 /* Builds the tree and the target groups for the current device state, as a
  * force evaluation does before its traversal, and copies out:
  *   groups[2 * g]  sorted body range [start, end) of each group, in body order;
- *   queue[2 * g]   the same groups in traversal order: eighth q of them is
- *                  XCD q's queue, its heavy groups (at least twice the mean
- *                  work of the last traversal) first;
+ *   queue[4 * k]   the traversal's queue entries (group start, group end,
+ *                  targets start, targets end): queue q holds entries
+ *                  [qbounds[q], qbounds[8 + q]), the parts of its fat groups
+ *                  (bit-exact FP64 only), then its heavy groups (at least
+ *                  twice the mean work of the last traversal), then the rest;
+ *   qbounds[16]    the queues' first entries, then their ends;
  *   cost[count]    per body (caller's index): the work units of its group in
  *                  the last traversal, which set that order (0 before any);
  *   order[count]   original index of each sorted position;
  *   *n_groups      the number of groups.
- * At most cap groups are copied; any output pointer may be NULL. */
+ * At most cap groups and cap queue entries are copied; any output pointer may
+ * be NULL. */
 int32_t cm_test_groups(void* engine, int32_t cap, int32_t* groups, int32_t* queue,
-                       int32_t* cost, int32_t* order, int32_t* n_groups);
+                       int32_t* qbounds, int32_t* cost, int32_t* order, int32_t* n_groups);
 
 #ifdef __cplusplus
 }

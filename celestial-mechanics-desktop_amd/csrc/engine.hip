@@ -323,7 +323,9 @@ int32_t cm_create(int32_t capacity, int32_t device, void** out_engine) {
     alloc((void**)&e->tree_dl, n * sizeof(uchar2));
     alloc((void**)&e->tree_tcount, (size_t)(cm::kMaxLevels + 1) * cm::tree_tiles((int)n) * 4);
     alloc((void**)&e->groups, n * sizeof(int2));
-    alloc((void**)&e->gq, n * sizeof(int2));
+    // queue entries: every group once, plus the parts of split groups
+    e->gq_cap = (int)std::min<size_t>(2 * n, (size_t)INT32_MAX / 2);
+    alloc((void**)&e->gq, (size_t)e->gq_cap * sizeof(int4));
     alloc((void**)&e->bcost, n * 4);
     alloc((void**)&e->gcost, n * 4);
     alloc((void**)&e->gwork, ((n + 1023) / 1024 + 1) * 8);

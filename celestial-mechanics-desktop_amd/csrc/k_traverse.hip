@@ -266,21 +266,43 @@ k_group_emit(int ntiles, int scanned, TreeHeader* __restrict__ hdr,
 // is stable, so each class keeps body order, and any bcost content (zeros for
 // a fresh engine: body order) gives a valid permutation; the order decides
 // which wavefront computes a group and when, never its result.
+//
+// Fat groups (split != 0: the bit-exact FP64 path, whose sums do not depend
+// on which lanes or wavefronts form them): a group whose work is at least
+// kFatFactor times the mean enters its queue, ahead of the heavy ones, as
+// P = min(kMaxParts, bodies, work / (2 mean)) entries, part p holding the
+// targets [x + p n / P, x + (p + 1) n / P) of its n bodies.  Each part walks
+// the whole group's box, so it finds the group's lists, and computes only its
+// targets (with the lanes of the other targets): a group spanning a wide box
+// (a sparse region of a thin disk) no longer sets the kernel's length on one
+// wavefront.  Entries are (group start, group end, targets start, targets
+// end); queue q fills [qbase[q], qend[q]) of the queue array, qbase[q] =
+// q ng / 8 + q B with B = (capacity - ng) / 8 spare entries per queue; a queue
+// whose parts would need more keeps its fat groups whole (as heavy ones).
 constexpr int kHeavyFactor = 2;
+constexpr int kFatFactor = 4;
+constexpr int kMaxParts = 8;
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderWaves = kOrderThreads / kWave;
+
+__device__ __forceinline__ int group_parts(long long c, long long mean, int nm, int split) {
+    if (!split || mean <= 0 || c < kFatFactor * mean || nm < 2) return 1;
+    return (int)min((long long)min(kMaxParts, nm), c / (2 * mean));
+}
 
 // one block per eighth; gcost: the work of each group in body order, twork
 // the totals per tile of bodies (both written by k_group_emit)
 __global__ void __launch_bounds__(kOrderThreads)
-k_group_order(int ntiles, const TreeHeader* __restrict__ hdr, const int2* __restrict__ groups,
-              const int32_t* __restrict__ gcost, const long long* __restrict__ twork,
-              int2* __restrict__ gq) {
-    __shared__ int s_cnt[kOrderWaves][2];   // per wave: heavy, light; count, then base
+k_group_order(int ntiles, int split, int qcap, TreeHeader* __restrict__ hdr,
+              const int2* __restrict__ groups, const int32_t* __restrict__ gcost,
+              const long long* __restrict__ twork, int4* __restrict__ gq) {
+    __shared__ int s_cnt[kOrderWaves][3];   // per wave: part entries, heavy, light; count, then base
+    __shared__ int s_fat[kOrderWaves];      // per wave: groups entered as parts
     __shared__ long long s_w[kOrderWaves];
+    __shared__ int s_split;
     const int ng = hdr->n_groups;
     const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
-    // twice the mean work over all groups (each counted as at least 1)
+    // the mean work over all groups (each counted as at least 1)
     long long sum = 0;
     for (int k = t; k < ntiles; k += kOrderThreads) sum += twork[k];
     for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
@@ -288,36 +310,88 @@ k_group_order(int ntiles, const TreeHeader* __restrict__ hdr, const int2* __rest
     __syncthreads();
     long long total = 0;
     for (int k = 0; k < kOrderWaves; k++) total += s_w[k];
-    const long long heavy = ng > 0 ? kHeavyFactor * total / ng : 0;
+    const long long mean = ng > 0 ? total / ng : 0;
+    const long long heavy = kHeavyFactor * mean;
     // wave w partitions a contiguous slice of the eighth
     const int lo = (int)((long long)blockIdx.x * ng / kXcds);
     const int len = (int)((long long)(blockIdx.x + 1) * ng / kXcds) - lo;
+    const int spare = split ? max(0, (qcap - ng) / kXcds) : 0;
+    const int qb = lo + (int)blockIdx.x * spare;
     const int ws = lo + (int)((long long)len * w / kOrderWaves);
     const int we = lo + (int)((long long)len * (w + 1) / kOrderWaves);
-    int nh = 0;
-    for (int i = ws + lane; i < we; i += kWave) nh += gcost[i] >= heavy ? 1 : 0;
-    for (int o = kWave / 2; o > 0; o >>= 1) nh += __shfl_xor(nh, o, kWave);
-    if (lane == 0) { s_cnt[w][0] = nh; s_cnt[w][1] = (we - ws) - nh; }
+    int np = 0, nf = 0, nh = 0;
+    for (int i = ws + lane; i < we; i += kWave) {
+        const int2 gv = groups[i];
+        const int P = group_parts(gcost[i], mean, gv.y - gv.x, split);
+        if (P > 1) { np += P; nf++; }
+        else nh += gcost[i] >= heavy ? 1 : 0;
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        np += __shfl_xor(np, o, kWave);
+        nf += __shfl_xor(nf, o, kWave);
+        nh += __shfl_xor(nh, o, kWave);
+    }
+    if (lane == 0) {
+        s_cnt[w][0] = np; s_cnt[w][1] = nh; s_cnt[w][2] = (we - ws) - nh - nf;
+        s_fat[w] = nf;
+    }
     __syncthreads();
     if (t == 0) {
-        // bases: the heavy groups of every slice, then the light ones
-        int run = lo;
-        for (int k = 0; k < 2; k++)
+        int extra = 0;
+        for (int u = 0; u < kOrderWaves; u++) extra += s_cnt[u][0] - s_fat[u];
+        const int ok = extra <= spare;
+        if (!ok)   // not enough room: the fat groups stay whole, as heavy ones
+            for (int u = 0; u < kOrderWaves; u++) {
+                s_cnt[u][1] += s_fat[u];
+                s_cnt[u][0] = 0;
+            }
+        s_split = ok;
+        // bases: the parts of every slice, then its heavy groups, then the
+        // light ones
+        int run = qb;
+        for (int k = 0; k < 3; k++)
             for (int u = 0; u < kOrderWaves; u++) {
                 const int c = s_cnt[u][k];
                 s_cnt[u][k] = run;
                 run += c;
             }
+        hdr->qbase[blockIdx.x] = qb;
+        hdr->qend[blockIdx.x] = run;
     }
     __syncthreads();
-    int hb = s_cnt[w][0], lb = s_cnt[w][1];
+    const int sp = s_split ? split : 0;
+    int pb = s_cnt[w][0], hb = s_cnt[w][1], lb = s_cnt[w][2];
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
     for (int b = ws; b < we; b += kWave) {
         const int i = b + lane;
         const bool ok = i < we;
-        const bool hv = ok && gcost[i] >= heavy;
-        const uint64_t mh = __ballot(hv), ml = __ballot(ok && !hv);
-        if (ok) gq[hv ? hb + __popcll(mh & lt) : lb + __popcll(ml & lt)] = groups[i];
+        int2 gv = make_int2(0, 0);
+        int P = 1;
+        if (ok) {
+            gv = groups[i];
+            P = group_parts(gcost[i], mean, gv.y - gv.x, sp);
+        }
+        const bool fat = ok && P > 1;
+        const bool hv = ok && !fat && gcost[i] >= heavy;
+        const uint64_t mh = __ballot(hv), ml = __ballot(ok && !fat && !hv);
+        // parts: an exclusive prefix sum of P over the fat lanes
+        int pin = fat ? P : 0;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int y = __shfl_up(pin, o, kWave);
+            if (lane >= o) pin += y;
+        }
+        const int ptot = __shfl(pin, kWave - 1, kWave);
+        if (fat) {
+            const int nm = gv.y - gv.x;
+            const int at = pb + pin - P;
+            for (int p = 0; p < P; p++)
+                gq[at + p] = make_int4(gv.x, gv.y, gv.x + (int)((long long)p * nm / P),
+                                       gv.x + (int)((long long)(p + 1) * nm / P));
+        } else if (ok) {
+            gq[hv ? hb + __popcll(mh & lt) : lb + __popcll(ml & lt)] =
+                make_int4(gv.x, gv.y, gv.x, gv.y);
+        }
+        pb += ptot;
         hb += __popcll(mh);
         lb += __popcll(ml);
     }
@@ -329,7 +403,7 @@ struct TravArgs {
     const int4* info;
     const double4* cm;
     const int32_t* order;
-    const int2* groups;      // in traversal order (k_group_order)
+    const int4* queue;       // queue entries in traversal order (k_group_order)
     int32_t* bcost;          // per body (caller's index): its group's work units
     const double* sorted;    // x, y, z, m, r blocks of `cap`
     int cap;
@@ -483,7 +557,6 @@ k_traverse(TravArgs a) {
     __shared__ float s_my[LEARNED ? kWave * kMlpUsed : 1];   // MLP outputs per cell
 
     const int lane = lane_id();
-    const int ngroups = a.hdr->n_groups;
     const double h = a.hdr->box[3];
     const int cap = a.cap;
     const double* sx = a.sorted;
@@ -499,14 +572,13 @@ k_traverse(TravArgs a) {
     // eighth q of the groups (in body order; heavy ones first) is queue q,
     // served first by the wavefronts of XCD q so that each XCD's L2 holds one
     // compact region of the tree; a wavefront whose queue is empty helps the
-    // others
+    // others.  A queue's bounds are read once per queue a wavefront serves.
     const int xcd = blockIdx.x % kXcds;
     int qi = 0;
+    int qlo = a.hdr->qbase[xcd], qhi = a.hdr->qend[xcd];
     for (;;) {
         const int q = (xcd + qi) % kXcds;
-        const int qhi = (int)((long long)(q + 1) * ngroups / kXcds);
-        if (lane == 0)
-            s_g = (int)((long long)q * ngroups / kXcds) + (int)atomicAdd(&a.hdr->work_next[q], 1u);
+        if (lane == 0) s_g = qlo + (int)atomicAdd(&a.hdr->work_next[q], 1u);
         __syncthreads();
         // group index, counts and list lengths are wave-uniform: keeping them
         // in scalar registers keeps the inner loops' control and LDS
@@ -515,21 +587,32 @@ k_traverse(TravArgs a) {
         __syncthreads();
         if (g >= qhi) {
             if (++qi == kXcds) break;
+            const int qn = (xcd + qi) % kXcds;
+            qlo = a.hdr->qbase[qn];
+            qhi = a.hdr->qend[qn];
             continue;
         }
-        const int2 gr = a.groups[g];
-        const int s = gr.x, e = gr.y;
+        // the group's bodies [gs, ge) set the box (and so the lists); the
+        // entry computes the targets [s, e) among them (all of them unless
+        // k_group_order split the group)
+        const int4 gr = a.queue[g];
+        const int gs = gr.x, ge = gr.y, s = gr.z, e = gr.w;
+        const bool whole = s == gs && e == ge;
         const int nmem = e - s;
         const int my = s + lane;
         const bool member = lane < nmem;
+        const bool inbox = lane < ge - gs;
+        double bxv = 0.0, byv = 0.0, bzv = 0.0;
+        if (inbox) { bxv = sx[gs + lane]; byv = sy[gs + lane]; bzv = sz[gs + lane]; }
         double xi = 0.0, yi = 0.0, zi = 0.0;
-        if (member) { xi = sx[my]; yi = sy[my]; zi = sz[my]; }
+        if (whole) { xi = bxv; yi = byv; zi = bzv; }
+        else if (member) { xi = sx[my]; yi = sy[my]; zi = sz[my]; }
 
         const double big = 1.7976931348623157e308;
-        double lo0 = wmin(member ? xi : big), lo1 = wmin(member ? yi : big),
-               lo2 = wmin(member ? zi : big);
-        double hi0 = wmax(member ? xi : -big), hi1 = wmax(member ? yi : -big),
-               hi2 = wmax(member ? zi : -big);
+        double lo0 = wmin(inbox ? bxv : big), lo1 = wmin(inbox ? byv : big),
+               lo2 = wmin(inbox ? bzv : big);
+        double hi0 = wmax(inbox ? bxv : -big), hi1 = wmax(inbox ? byv : -big),
+               hi2 = wmax(inbox ? bzv : -big);
         const double cxg = (lo0 + hi0) * 0.5;
         const double cyg = (lo1 + hi1) * 0.5;
         const double czg = (lo2 + hi2) * 0.5;
@@ -564,10 +647,17 @@ k_traverse(TravArgs a) {
         // W = C / kl for a list of C slots (128; 64 in the FP64 kernels),
         // so lane sub reads slots sub * W + t, t = 0, 1, ...
         // The FP64 baseline (FP64_FAST) splits small groups the same way.
+        // The bit-exact FP64 path (kOrdered) spreads a target's entries over
+        // kl lanes too, but only their terms: lane (t, q) forms the term of
+        // entry k kl + q in trip k, and target t's own lane subtracts the kl
+        // terms of the trip one after another, in entry order, so every sum
+        // takes the same operations in the same order as one lane alone.
         constexpr bool kSplit = (PREC == CM_PREC_FP32_SPLIT && !LEARNED) ||
                                 (PREC == CM_PREC_FP64_FAST && !POT);
+        constexpr bool kOrdered = PREC == CM_PREC_FP64 && !POT;
+        constexpr bool kLanes = kSplit || kOrdered;
         int kl = 1, klog = 0;
-        if constexpr (kSplit) {
+        if constexpr (kLanes) {
             while (2 * kl * nmem <= kWave) { kl *= 2; klog++; }
         }
         // the group's work in units of one list entry per lane (13 VALU
@@ -577,8 +667,8 @@ k_traverse(TravArgs a) {
         const int tl = lane % nmem, sub = lane / nmem;
         const bool worker = lane < kl * nmem;
         float txw = txf, tyw = tyf, tzw = tzf;
-        double xw = xi, yw = yi, zw = zi;   // FP64_FAST: the lane's target
-        if constexpr (kSplit) {
+        double xw = xi, yw = yi, zw = zi;   // FP64 paths: the lane's target
+        if constexpr (kLanes) {
             if (worker && sub > 0) {
                 const int mt = s + tl;
                 if constexpr (PREC == CM_PREC_FP32_SPLIT) {
@@ -691,7 +781,7 @@ k_traverse(TravArgs a) {
         };
 
         auto flush = [&](int cnt, bool cells) {
-            wcost += kSplit ? (cnt + kl - 1) >> klog : cnt;
+            wcost += kLanes ? (cnt + kl - 1) >> klog : cnt;
             // the list's kPL slots; W = kPL / kl
             const int wlog = (kPL == kWave ? 6 : 7) - klog;
             const int kw = kPL >> klog;
@@ -719,12 +809,12 @@ k_traverse(TravArgs a) {
 #pragma unroll
             for (int hh = 0; hh < kPL / kWave; hh++) {
                 const int j = hh * kWave + lane;
-                const int slot = kSplit ? (((j & (kl - 1)) << wlog) | (j >> klog)) : j;
+                const int slot = kLanes ? (((j & (kl - 1)) << wlog) | (j >> klog)) : j;
                 if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                     // the slots past cnt hold massless far entries, whose terms
                     // are exactly zero, so the loops below may run past cnt
                     if (j >= cnt) st.v[slot] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
-                } else if constexpr (kSplit) {
+                } else if constexpr (kLanes) {
                     if (j >= cnt) st.v[slot] = make_double4(1e18, 1e18, 1e18, 0.0);
                 }
                 if (j < cnt) {
@@ -756,7 +846,82 @@ k_traverse(TravArgs a) {
                               (unsigned long long)__popcll(selfm);
                 }
             }
-            if (kSplit ? worker : member) {
+            if constexpr (kOrdered) {
+              if (kl > 1) {
+                // every lane takes part (the terms are handed over by
+                // shuffles); lanes past kl * nmem and entries past cnt
+                // contribute exact zeros, as does a body's own entry, which
+                // the one-lane loop skips: x - (+0) == x for every x the sums
+                // can hold (they start at +0, so never hold -0).  Four trips'
+                // terms are formed before any is handed over, so the
+                // shuffles' latency is paid once per four trips.
+                const int trips = (cnt + kl - 1) >> klog;
+                const int mt = s + tl;
+                for (int t0 = 0; t0 < trips; t0 += 4) {
+                    double tx[4], ty[4], tz[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const int t = t0 + u;
+                        const int j = t * kl + sub;
+                        tx[u] = 0.0; ty[u] = 0.0; tz[u] = 0.0;
+                        if (worker && t < trips && j < cnt) {
+                            const double4 v = st.v[rbase + t];
+                            const double dx = xw - v.x, dy = yw - v.y, dz = zw - v.z;
+                            const double dist2 = dx * dx + dy * dy + dz * dz;
+                            double f = 0.0;
+                            if (cells) {
+                                const double soft = dist2 + a.eps2;
+                                const double inv = 1.0 / sqrt(soft);
+                                const double inv3 = inv * inv * inv;
+                                f = v.w * inv3;
+                            } else if (st.id[j] != mt) {
+                                f = gk_coeff(dist2, v.w, st.r[j], a.eps2, shell);
+                            }
+                            tx[u] = f * dx; ty[u] = f * dy; tz[u] = f * dz;
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        if (t0 + u >= trips) break;
+                        for (int q = 0; q < kl; q++) {
+                            double vx = tx[u], vy = ty[u], vz = tz[u];
+                            if (q > 0) {
+                                const int src = lane + q * nmem;
+                                const int sl = src < kWave ? src : lane;
+                                vx = __shfl(tx[u], sl, kWave);
+                                vy = __shfl(ty[u], sl, kWave);
+                                vz = __shfl(tz[u], sl, kWave);
+                            }
+                            if (cells) { acx -= vx; acy -= vy; acz -= vz; }
+                            else { apx -= vx; apy -= vy; apz -= vz; }
+                        }
+                    }
+                }
+              } else if (member) {
+                // one lane per target: the entries in order
+                if (cells) {
+                    for (int j = 0; j < cnt; j++) {
+                        double4 v = st.v[j];
+                        double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
+                        double dist2 = dx * dx + dy * dy + dz * dz;
+                        double soft = dist2 + a.eps2;
+                        double inv = 1.0 / sqrt(soft);
+                        double inv3 = inv * inv * inv;
+                        double f = v.w * inv3;
+                        acx -= f * dx; acy -= f * dy; acz -= f * dz;
+                    }
+                } else {
+                    for (int j = 0; j < cnt; j++) {
+                        if (st.id[j] == my) continue;
+                        double4 v = st.v[j];
+                        double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
+                        double dist2 = dx * dx + dy * dy + dz * dz;
+                        double f = gk_coeff(dist2, v.w, st.r[j], a.eps2, shell);
+                        apx -= f * dx; apy -= f * dy; apz -= f * dz;
+                    }
+                }
+              }
+            } else if (kSplit ? worker : member) {
                 if constexpr (PREC == CM_PREC_FP32_SPLIT) {
                     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
                     const float e2 = (float)a.eps2;
@@ -845,26 +1010,6 @@ k_traverse(TravArgs a) {
                         }
                     }
                     apx += gx; apy += gy; apz += gz;
-                } else if (cells) {
-                    for (int j = 0; j < cnt; j++) {
-                        double4 v = st.v[j];
-                        double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
-                        double dist2 = dx * dx + dy * dy + dz * dz;
-                        double soft = dist2 + a.eps2;
-                        double inv = 1.0 / sqrt(soft);
-                        double inv3 = inv * inv * inv;
-                        double f = v.w * inv3;
-                        acx -= f * dx; acy -= f * dy; acz -= f * dz;
-                    }
-                } else {
-                    for (int j = 0; j < cnt; j++) {
-                        if (st.id[j] == my) continue;
-                        double4 v = st.v[j];
-                        double dx = xi - v.x, dy = yi - v.y, dz = zi - v.z;
-                        double dist2 = dx * dx + dy * dy + dz * dz;
-                        double f = gk_coeff(dist2, v.w, st.r[j], a.eps2, shell);
-                        apx -= f * dx; apy -= f * dy; apz -= f * dz;
-                    }
                 }
             }
             if constexpr (LEARNED) {
@@ -1077,7 +1222,10 @@ k_traverse(TravArgs a) {
         }
         if (member) {
             const int o = a.order[my];
-            a.bcost[o] = wcost;
+            // a part records the whole group's work (the parts' share of
+            // the targets scaled up), which the next order reads
+            a.bcost[o] = whole ? wcost : (int)min((long long)wcost * (ge - gs) / nmem,
+                                                   (long long)INT32_MAX);
             if constexpr (POT) {
                 a.rows[o] = 0.5 * (sm[my] * (acx + apx));
             } else if constexpr (PREC == CM_PREC_FP32_SPLIT) {
@@ -1122,8 +1270,11 @@ int form_groups(Engine& e, const int32_t* order) {
     hipLaunchKernelGGL(k_group_emit, dim3(tiles), dim3(kGroupTile), 0, e.stream, tiles,
                        scanned, e.hdr, e.gmark, e.gtile, order, e.bcost, e.groups, e.gcost,
                        e.gwork);
+    // parts of fat groups only for the bit-exact FP64 path, whose sums they
+    // leave unchanged
+    const int split = e.cfg.precision == CM_PREC_FP64 ? 1 : 0;
     hipLaunchKernelGGL(k_group_order, dim3(kXcds), dim3(kOrderThreads), 0, e.stream, tiles,
-                       e.hdr, e.groups, e.gcost, e.gwork, e.gq);
+                       split, e.gq_cap, e.hdr, e.groups, e.gcost, e.gwork, e.gq);
     CM_HIP_TRY(e, hipGetLastError());
     return CM_OK;
 }
@@ -1154,7 +1305,7 @@ int launch_barnes_hut(Engine& e, double softening) {
     a.info = e.nd_info;
     a.cm = e.nd_cm;
     a.order = order;
-    a.groups = e.gq;
+    a.queue = e.gq;
     a.bcost = e.bcost;
     a.sorted = e.sorted;
     a.cap = e.capacity;
@@ -1217,7 +1368,7 @@ int launch_bh_potential(Engine& e, double softening, double* rows) {
     a.info = e.nd_info;
     a.cm = e.nd_cm;
     a.order = order;
-    a.groups = e.gq;
+    a.queue = e.gq;
     a.bcost = e.bcost;
     a.sorted = e.sorted;
     a.cap = e.capacity;
@@ -1239,7 +1390,8 @@ int launch_bh_potential(Engine& e, double softening, double* rows) {
 }  // namespace cm
 
 extern "C" int32_t cm_test_groups(void* h, int32_t cap, int32_t* groups, int32_t* queue,
-                                  int32_t* cost, int32_t* order, int32_t* n_groups) {
+                                  int32_t* qbounds, int32_t* cost, int32_t* order,
+                                  int32_t* n_groups) {
     using namespace cm;
     Engine* e = static_cast<Engine*>(h);
     if (!e) return CM_ERR_NOT_INITIALIZED;
@@ -1263,9 +1415,16 @@ extern "C" int32_t cm_test_groups(void* h, int32_t cap, int32_t* groups, int32_t
     if (groups && m)
         CM_HIP_TRY(*e, hipMemcpyAsync(groups, e->groups, m * sizeof(int2), hipMemcpyDeviceToHost,
                                       e->stream));
-    if (queue && m)
-        CM_HIP_TRY(*e, hipMemcpyAsync(queue, e->gq, m * sizeof(int2), hipMemcpyDeviceToHost,
+    // the queue: entries [0, qend[7]) of which queue q holds [qbase[q], qend[q])
+    const size_t mq = (size_t)std::min(cap, std::min(hh.qend[kXcds - 1], e->gq_cap));
+    if (queue && mq)
+        CM_HIP_TRY(*e, hipMemcpyAsync(queue, e->gq, mq * sizeof(int4), hipMemcpyDeviceToHost,
                                       e->stream));
+    if (qbounds)
+        for (int q = 0; q < kXcds; q++) {
+            qbounds[q] = hh.qbase[q];
+            qbounds[kXcds + q] = hh.qend[q];
+        }
     if (cost)
         CM_HIP_TRY(*e, hipMemcpyAsync(cost, e->bcost, (size_t)n * 4, hipMemcpyDeviceToHost,
                                       e->stream));
