@@ -986,6 +986,12 @@ int collide_compact(Engine& e, int32_t* new_count, int32_t* old_index) {
     int32_t* t32 = reinterpret_cast<int32_t*>(c->ka);
     hipLaunchKernelGGL(k_scatter<int32_t>, dim3(nb), dim3(256), 0, st, n, c->new_index, e.type, t32);
     CM_HIP_TRY(e, hipMemcpyAsync(e.type, t32, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    // the traversal's per-body work counts follow their bodies, so the next
+    // evaluation's group order (heavy groups first, fat groups in parts)
+    // reads each group's own work
+    hipLaunchKernelGGL(k_scatter<int32_t>, dim3(nb), dim3(256), 0, st, n, c->new_index, e.bcost,
+                       t32);
+    CM_HIP_TRY(e, hipMemcpyAsync(e.bcost, t32, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     int32_t* old = c->va;
     hipLaunchKernelGGL(k_old_index, dim3(nb), dim3(256), 0, st, n, c->new_index, old);
     CM_HIP_TRY(e, hipGetLastError());
