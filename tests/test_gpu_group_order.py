@@ -149,6 +149,12 @@ def test_fat_groups_split_into_parts_keep_fp64_bit_exact():
     for out in runs:
         for f in ("acc_x", "acc_y", "acc_z"):
             np.testing.assert_array_equal(getattr(out, f)[:n], getattr(fresh, f)[:n])
+    # the Barnes-Hut potential rows walk the same queue (parts included)
+    rows = be.potential_rows(EPS, method=2)
+    clean = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5, leaf_capacity=8)
+    clean.upload(soa)
+    np.testing.assert_array_equal(rows, clean.potential_rows(EPS, method=2))
+    clean.close()
     be.close()
 
 
