@@ -304,7 +304,8 @@ def disk(args, cmhip):
     1e-6 Msun, radii from 3 g/cm^3 inflated 100x), FP64 state; a cycle is
     one Verlet step (FP64 Barnes-Hut, theta 0.5, the bit-exact path), collision
     detection, resolution in --collision-mode (MergeOnly by default) and
-    compaction."""
+    compaction (in place on the device; the survivors' old indices stay
+    there, as the cycle does not use them)."""
     eps, dt = 1e-4, 1e-3
     n = args.n if args.n_set else 262144
     soa = cmhip.generate_disk(n, seed=42, a_min=0.5, a_max=5.0, sigma_e=0.02, sigma_i=0.01,
@@ -316,12 +317,13 @@ def disk(args, cmhip):
             "realistic": cmhip.COLLISION_REALISTIC}[args.collision_mode]
     be.configure_collisions(mode=mode)
     be.upload(soa)
+    warm_clocks(be, eps)
 
     def cycle():
         be.step(eps, dt, 1)
         _, ne = be.detect_collisions(0)
         m = be.resolve_collisions()
-        be.compact()
+        be.compact(with_old_index=False)   # the cycle needs no host copy of the index map
         return ne, m
 
     for _ in range(args.warmup):

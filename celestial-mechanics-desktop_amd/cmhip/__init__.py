@@ -633,10 +633,15 @@ class HipPhysicsBackend:
         self._check(self._L.cm_resolve_collisions(self._h, C.byref(n)), "cm_resolve_collisions")
         return n.value
 
-    def compact(self):
-        n0 = self.count
-        old = np.zeros(max(n0, 1), dtype=np.int32)
+    def compact(self, with_old_index=True):
+        """Removes the inactive bodies in place; returns the new count and,
+        with with_old_index, each survivor's index before the call (a host
+        copy of n int32 values; None without it)."""
         n = C.c_int32()
+        if not with_old_index:
+            self._check(self._L.cm_compact(self._h, C.byref(n), None), "cm_compact")
+            return n.value, None
+        old = np.zeros(max(self.count, 1), dtype=np.int32)
         self._check(self._L.cm_compact(self._h, C.byref(n), old.ctypes.data), "cm_compact")
         return n.value, old[:n.value].copy()
 

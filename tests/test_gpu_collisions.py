@@ -74,23 +74,28 @@ def test_resolve_and_compact_bit_exact():
 
 def test_collision_cycle_with_barnes_hut_bit_exact():
     """Config-3-like cycle: Verlet with FP64 Barnes-Hut, detect, merge, compact,
-    repeated; the final state equals the oracle's sequence bit for bit."""
+    repeated; the final state equals the oracle's sequence bit for bit (every
+    other compaction leaves the survivors' old indices on the device)."""
     soa = crowded_disk(8000, seed=7)
     s = soa_to_state(soa)
     be = cmhip.HipPhysicsBackend(soa.count, precision=cmhip.PREC_FP64, theta=0.5)
     be.upload(soa)
     eps, dt = 1e-4, 1e-3
     total_merged = 0
-    for _ in range(4):
+    for k in range(4):
         be.step(eps, dt, 1)
         be.detect_collisions(0)
         m = be.resolve_merges()
         total_merged += m
-        be.compact()
+        n_new, old = be.compact(with_old_index=k % 2 == 0)
+        assert (old is None) == (k % 2 == 1)
         O.verlet_step(s, dt, lambda st: O.bh_tree(st, eps, 0.5, levels=10, leaf_cap=16,
                                                   group_size=64)[0])
         assert O.resolve_merge(O.detect(s), s) == m
-        s, _ = O.compact(s)
+        s, idx = O.compact(s)
+        assert n_new == len(idx)
+        if old is not None:
+            np.testing.assert_array_equal(old, idx)
     out = cmhip.BodySoA(soa.count)
     be.download(out)
     assert out.count == len(s["px"]) and total_merged > 0
