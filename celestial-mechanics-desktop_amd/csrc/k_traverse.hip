@@ -284,6 +284,10 @@ constexpr int kFatFactor = 4;
 constexpr int kMaxParts = 8;
 constexpr int kOrderThreads = 1024;
 constexpr int kOrderWaves = kOrderThreads / kWave;
+// opened leaves of at most this many bodies are copied into the body list by
+// their own lanes; a walk step that opens a larger one expands its leaves 64
+// list positions at a time instead
+constexpr int kDirectLeaf = 16;
 
 __device__ __forceinline__ int group_parts(long long c, long long mean, int nm, int split) {
     if (!split || mean <= 0 || c < kFatFactor * mean || nm < 2) return 1;
@@ -1113,8 +1117,14 @@ k_traverse(TravArgs a) {
             int nl = kind == 2 ? nd.w : 0;
             int lin = wscan_incl(nl);
             int ltot = __builtin_amdgcn_readlane(lin, kWave - 1);
-            s_lpref[lane] = lin - nl;
-            s_lstart[lane] = nd.z;
+            // when every opened leaf holds at most kDirectLeaf bodies (all but
+            // the deepest level's oversized leaves), each lane copies its own
+            // leaf's bodies into their list positions
+            const bool direct = __ballot(nl > kDirectLeaf) == 0;
+            if (!direct) {
+                s_lpref[lane] = lin - nl;
+                s_lstart[lane] = nd.z;
+            }
             __syncthreads();
             if constexpr (kFar) {
                 if (nfar >= kPL) {
@@ -1141,7 +1151,23 @@ k_traverse(TravArgs a) {
             // each leaf marks its first position with its lane, and a running
             // maximum over the window (with the owner of the position before
             // it carried in) gives every position its leaf
-            int done = 0, carry = -1;
+            if (direct && ltot > 0) {
+                // positions [done, done + tk) of this step's leaf bodies fill
+                // list slots [npart, npart + tk); a full list is flushed
+                const int pref = lin - nl;
+                int done = 0;
+                while (done < ltot) {
+                    const int tk = min(ltot - done, kPL - npart);
+                    const int c0 = max(0, done - pref), c1 = min(nl, done + tk - pref);
+                    const int at = npart + pref - done;
+                    for (int c = c0; c < c1; c++) s_part[at + c] = nd.z + c;
+                    npart += tk;
+                    done += tk;
+                    __syncthreads();
+                    if (npart == kPL) { flush(kPL, false); npart = 0; }
+                }
+            }
+            int done = direct ? ltot : 0, carry = -1;
             while (done < ltot) {
                 // a window covers at most 64 positions (one per lane)
                 const int room = min(kPL - npart, kWave);
