@@ -19,6 +19,7 @@ if the library is missing or no GPU is present the calls raise.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -762,6 +763,118 @@ class CollisionResolver:
         else:
             self.last_collision_count = backend.resolve_merges()
         return self.last_collision_count
+
+
+class DiagnosticThresholds:
+    """DiagnosticThresholds (engine/include/celestial/profile/energy_tracker.hpp:14-19)."""
+
+    def __init__(self, max_energy_drift=0.001, max_momentum_drift=1e-8,
+                 max_angular_momentum_drift=1e-8, max_com_velocity_drift=1e-10):
+        self.max_energy_drift = max_energy_drift
+        self.max_momentum_drift = max_momentum_drift
+        self.max_angular_momentum_drift = max_angular_momentum_drift
+        self.max_com_velocity_drift = max_com_velocity_drift
+
+
+class EnergyTracker:
+    """EnergyTracker (engine/src/profile/energy_tracker.cpp): snapshots of a
+    backend's device state and their drift against the first one recorded.
+
+    compute() takes one cm_energy call (compensated sums on the GPU; the
+    potential is the exact pair sum, with_potential=1, or the Barnes-Hut
+    potential of compute_with_bh, with_potential=2) and derives the snapshot's
+    other fields the way energy_tracker.cpp:65-92 does: |p| and |L|, the centre
+    of mass, its velocity (the momentum over the total mass: the engine's
+    com_mv sums are the momentum sums), the virial ratio 2 KE / |PE|.
+    record(), the drifts, check_diagnostics() and the 300-entry rolling window
+    follow energy_tracker.cpp:169-262 (host bookkeeping, no device work)."""
+
+    ROLLING_WINDOW = 300
+
+    def __init__(self):
+        self.reset()
+
+    def reset(self):
+        self.initial = None
+        self.current = None
+        self.accumulated_error = 0.0
+        self._energy = [0.0] * self.ROLLING_WINDOW
+        self._drift = [0.0] * self.ROLLING_WINDOW
+        self._count = 0
+        self._index = 0
+
+    @property
+    def has_initial(self):
+        return self.initial is not None
+
+    @staticmethod
+    def compute(backend, softening, with_potential=1, step_number=0):
+        e = backend.energy(softening, with_potential=with_potential)
+        ke, pe = e["kinetic"], e["potential"]
+        p, lv, m = e["momentum"], e["angular_momentum"], e["total_mass"]
+        snap = {"kinetic_energy": ke, "potential_energy": pe, "total_energy": ke + pe,
+                "momentum": tuple(p), "momentum_magnitude": math.sqrt(p[0] * p[0] + p[1] * p[1] +
+                                                                      p[2] * p[2]),
+                "angular_momentum": tuple(lv), "angular_momentum_magnitude":
+                    math.sqrt(lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2]),
+                "total_mass": m, "com": (0.0, 0.0, 0.0), "com_velocity": (0.0, 0.0, 0.0),
+                "step_number": int(step_number)}
+        if m > 0.0:
+            snap["com"] = tuple(e["center_of_mass"])
+            snap["com_velocity"] = (p[0] / m, p[1] / m, p[2] / m)
+        snap["virial_ratio"] = 2.0 * ke / abs(pe) if pe != 0.0 else 0.0
+        return snap
+
+    def record(self, snap):
+        if self.initial is None:
+            self.initial = snap
+        self.current = snap
+        drift = abs(self.energy_drift())
+        self.accumulated_error += drift
+        self._energy[self._index] = snap["total_energy"]
+        self._drift[self._index] = drift
+        self._index = (self._index + 1) % self.ROLLING_WINDOW
+        self._count = min(self._count + 1, self.ROLLING_WINDOW)
+
+    def energy_drift(self):
+        if self.initial is None or self.initial["total_energy"] == 0.0:
+            return 0.0
+        e0 = self.initial["total_energy"]
+        return (self.current["total_energy"] - e0) / abs(e0)
+
+    def _vec_drift(self, key):
+        if self.initial is None:
+            return 0.0
+        a, b = self.current[key], self.initial[key]
+        d = (a[0] - b[0], a[1] - b[1], a[2] - b[2])
+        return math.sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])
+
+    def momentum_drift(self):
+        return self._vec_drift("momentum")
+
+    def angular_momentum_drift(self):
+        return self._vec_drift("angular_momentum")
+
+    def com_position_drift(self):
+        return self._vec_drift("com")
+
+    def com_velocity_drift(self):
+        return self._vec_drift("com_velocity")
+
+    def check_diagnostics(self, thresholds=None):
+        t = thresholds or DiagnosticThresholds()
+        if self.initial is None:
+            return True
+        return not (abs(self.energy_drift()) > t.max_energy_drift or
+                    self.momentum_drift() > t.max_momentum_drift or
+                    self.angular_momentum_drift() > t.max_angular_momentum_drift or
+                    self.com_velocity_drift() > t.max_com_velocity_drift)
+
+    def rolling_avg_energy(self):
+        return sum(self._energy[:self._count]) / self._count if self._count else 0.0
+
+    def rolling_avg_drift(self):
+        return sum(self._drift[:self._count]) / self._count if self._count else 0.0
 
 
 def generate_ensemble(n_systems, seed=42, n_min=8, n_max=2048, m_min=1e-7, m_max=1e-3,

@@ -90,3 +90,51 @@ def test_simulation_state_with_collisions():
     for got, ref in zip(*runs):
         for k, v in ref.items():
             assert got[k] == v, (k, got[k], v)
+
+
+def test_energy_tracker_over_device_steps(plummer):
+    """EnergyTracker (engine/src/profile/energy_tracker.cpp) over device steps:
+    each snapshot's energies, momentum and centre of mass against the oracle's
+    exact sums of the downloaded state, and the tracker's drift equal to the
+    SimulationState's of the same steps (both relative to the first step)."""
+    n = 3000
+    soa = plummer(n, cut=0.999)
+    soa.is_active[:n][::50] = 0
+    be = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5)
+    be.upload(soa)
+    tr = cmhip.EnergyTracker()
+    for k in range(5):
+        be.nbody_step(EPS, DT, 1, collisions=False)
+        tr.record(cmhip.EnergyTracker.compute(be, EPS, with_potential=1, step_number=k + 1))
+        out = cmhip.BodySoA(n)
+        be.download(out)
+        s = soa_to_state(out)
+        x = O.energy_exact(s, EPS, True, nthreads=8)
+        cur = tr.current
+        assert abs(cur["kinetic_energy"] - x["ke"]) <= 1e-13 * abs(x["ke"])
+        assert abs(cur["potential_energy"] - x["pe"]) <= 1e-13 * abs(x["pe"])
+        assert abs(cur["virial_ratio"] - 2 * x["ke"] / abs(x["pe"])) <= 1e-12
+        cx, cy, cz, m = O.center_of_mass(s)
+        assert abs(cur["total_mass"] - m) <= 1e-14 * m
+        for got, want in zip(cur["com"], (cx, cy, cz)):
+            assert abs(got - want) <= 1e-13
+        p = (x["px"], x["py"], x["pz"])
+        for got, want in zip(cur["momentum"], p):
+            assert abs(got - want) <= 1e-14
+        for got, want in zip(cur["com_velocity"], p):
+            assert abs(got - want / m) <= 1e-14
+        if k == 0:
+            p0 = p
+    dp = np.array(p) - np.array(p0)
+    assert abs(tr.momentum_drift() - float(np.sqrt((dp * dp).sum()))) <= 1e-14
+    be2 = cmhip.HipPhysicsBackend(n, precision=cmhip.PREC_FP64, theta=0.5)
+    be2.upload(soa)
+    states = be2.nbody_step_diag(EPS, DT, 5, collisions=False, with_potential=1)
+    assert tr.energy_drift() == states[-1]["energy_drift"]
+    assert tr.initial["step_number"] == 1 and tr.current["step_number"] == 5
+    loose = cmhip.DiagnosticThresholds(max_energy_drift=1e-3, max_momentum_drift=1e-5,
+                                       max_angular_momentum_drift=1e-5,
+                                       max_com_velocity_drift=1e-5)
+    assert tr.energy_drift() != 0.0 and tr.check_diagnostics(loose)
+    loose.max_energy_drift = abs(tr.energy_drift()) / 2
+    assert not tr.check_diagnostics(loose)
