@@ -1146,11 +1146,6 @@ k_traverse(TravArgs a) {
                 ncell = rest;
                 __syncthreads();
             }
-            // position q of the opened leaves' bodies belongs to the last lane
-            // whose leaf starts at or before q: per window of 64 positions,
-            // each leaf marks its first position with its lane, and a running
-            // maximum over the window (with the owner of the position before
-            // it carried in) gives every position its leaf
             if (direct && ltot > 0) {
                 // positions [done, done + tk) of this step's leaf bodies fill
                 // list slots [npart, npart + tk); a full list is flushed
@@ -1167,6 +1162,11 @@ k_traverse(TravArgs a) {
                     if (npart == kPL) { flush(kPL, false); npart = 0; }
                 }
             }
+            // otherwise position q of the opened leaves' bodies belongs to the last lane
+            // whose leaf starts at or before q: per window of 64 positions,
+            // each leaf marks its first position with its lane, and a running
+            // maximum over the window (with the owner of the position before
+            // it carried in) gives every position its leaf
             int done = direct ? ltot : 0, carry = -1;
             while (done < ltot) {
                 // a window covers at most 64 positions (one per lane)
