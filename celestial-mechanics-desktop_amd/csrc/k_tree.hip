@@ -47,12 +47,30 @@ k_bbox(int n, const double* __restrict__ px, const double* __restrict__ py,
     double mn0 = 1.7976931348623157e308, mn1 = mn0, mn2 = mn0;
     double mx0 = -mn0, mx1 = -mn0, mx2 = -mn0;
     int c = 0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        if (!act[i]) continue;
-        double x = px[i], y = py[i], z = pz[i];
-        mn0 = x < mn0 ? x : mn0; mn1 = y < mn1 ? y : mn1; mn2 = z < mn2 ? z : mn2;
-        mx0 = x > mx0 ? x : mx0; mx1 = y > mx1 ? y : mx1; mx2 = z > mx2 ? z : mx2;
-        c++;
+    const long long stride = (long long)gridDim.x * 256;
+    for (long long i0 = (long long)blockIdx.x * 256 + threadIdx.x; i0 < n; i0 += 4 * stride) {
+        // the loads of a thread's next four bodies are issued before any is
+        // folded; the fold takes them in index order, as a plain loop would
+        uint8_t a[4];
+        double x[4], y[4], z[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const long long i = i0 + u * stride;
+            const bool in = i < n;
+            a[u] = in ? act[i] : 0;
+            x[u] = in ? px[i] : 0.0;
+            y[u] = in ? py[i] : 0.0;
+            z[u] = in ? pz[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (!a[u]) continue;
+            mn0 = x[u] < mn0 ? x[u] : mn0; mn1 = y[u] < mn1 ? y[u] : mn1;
+            mn2 = z[u] < mn2 ? z[u] : mn2;
+            mx0 = x[u] > mx0 ? x[u] : mx0; mx1 = y[u] > mx1 ? y[u] : mx1;
+            mx2 = z[u] > mx2 ? z[u] : mx2;
+            c++;
+        }
     }
     int t = threadIdx.x;
     s[0][t] = mn0; s[1][t] = mn1; s[2][t] = mn2;
